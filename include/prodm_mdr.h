@@ -1,0 +1,168 @@
+/*
+ * prodm_mdr.h -- C ABI of the MI355X (gfx950) MDR refactor/retrieve library
+ * (libprodm_mdr.so). Plain pointers and sizes only; device pointers are HIP
+ * device memory, `stream` is a hipStream_t (NULL = default stream). Every
+ * entry point is asynchronous on `stream` unless stated otherwise, returns
+ * PRODM_OK (0) or a negative status, and never exits the process; the
+ * message of the last failure on the calling thread is prodm_last_error().
+ *
+ * Each entry point names the ProDM interface it replaces (paths relative to
+ * the ProDM repository root). The C++ host classes in include/prodm/ (header-only)
+ * wrap these calls behind the same component concepts that ComposedRefactor
+ * (include/MDR/Refactor/ComposedRefactor.hpp:15-19) and ComposedReconstructor
+ * (include/MDR/Reconstructor/ComposedReconstructor.hpp:17-21) are templated on.
+ */
+#ifndef PRODM_MDR_H
+#define PRODM_MDR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PRODM_OK 0
+#define PRODM_ERR_ARG (-1)
+#define PRODM_ERR_GEOMETRY (-2)
+#define PRODM_ERR_HIP (-3)
+#define PRODM_ERR_NOMEM (-4)
+#define PRODM_ERR_STATE (-5)
+#define PRODM_ERR_UNSUPPORTED (-6)
+
+#define PRODM_F32 0
+#define PRODM_F64 1
+#define PRODM_ENC_NEGABINARY 0
+#define PRODM_ENC_GROUPED 1
+
+typedef void *prodm_stream_t;
+
+int prodm_version(void);
+const char *prodm_last_error(void);
+
+/* Level geometry, RefactorUtils.hpp:21-59 (compute_level_dims /
+ * compute_level_elements). out_dims: (L+1)*ndims, level 0 coarsest.
+ * Returns 1 when every dim > 1 stays odd down to level 0 (the supported
+ * decomposition geometry), 0 otherwise; host only.                       */
+int prodm_level_geometry(const uint32_t *dims, int ndims, int target_level, uint32_t *out_dims,
+                         uint64_t *out_elements);
+
+/* ---- Decomposer: MGARDOrthoganalDecomposer<T>::decompose / recompose
+ *      (include/MDR/Decomposer/MGARD.hpp:14-35). Out of place on device
+ *      arrays with the same dims/strides; strides NULL = dense.          */
+size_t prodm_mgard_workspace_bytes(const uint32_t *dims, int ndims, int target_level, int dtype);
+int prodm_decompose_f32(const float *d_in, float *d_out, const uint32_t *dims, int ndims, int target_level,
+                        const uint32_t *strides, void *d_ws, size_t ws_bytes, prodm_stream_t stream);
+int prodm_decompose_f64(const double *d_in, double *d_out, const uint32_t *dims, int ndims, int target_level,
+                        const uint32_t *strides, void *d_ws, size_t ws_bytes, prodm_stream_t stream);
+int prodm_recompose_f32(const float *d_in, float *d_out, const uint32_t *dims, int ndims, int target_level,
+                        const uint32_t *strides, void *d_ws, size_t ws_bytes, prodm_stream_t stream);
+int prodm_recompose_f64(const double *d_in, double *d_out, const uint32_t *dims, int ndims, int target_level,
+                        const uint32_t *strides, void *d_ws, size_t ws_bytes, prodm_stream_t stream);
+
+/* ---- Interleaver: DirectInterleaver<T>::interleave / reposition
+ *      (include/MDR/Interleaver/DirectInterleaver.hpp:12-63); ndims 1..3. */
+int prodm_interleave_f32(const float *d_data, const uint32_t *dims, int ndims, const uint32_t *dims_fine,
+                         const uint32_t *dims_coarse, const uint32_t *strides, float *d_buffer, prodm_stream_t stream);
+int prodm_interleave_f64(const double *d_data, const uint32_t *dims, int ndims, const uint32_t *dims_fine,
+                         const uint32_t *dims_coarse, const uint32_t *strides, double *d_buffer, prodm_stream_t stream);
+int prodm_reposition_f32(const float *d_buffer, const uint32_t *dims, int ndims, const uint32_t *dims_fine,
+                         const uint32_t *dims_coarse, const uint32_t *strides, float *d_data, prodm_stream_t stream);
+int prodm_reposition_f64(const double *d_buffer, const uint32_t *dims, int ndims, const uint32_t *dims_fine,
+                         const uint32_t *dims_coarse, const uint32_t *strides, double *d_data, prodm_stream_t stream);
+
+/* ---- compute_max_abs_value (RefactorUtils.hpp:68-76). Synchronizes the
+ *      stream and returns the value on the host.                        */
+int prodm_level_maxabs_f32(const float *d_x, uint64_t n, float *h_out, prodm_stream_t stream);
+int prodm_level_maxabs_f64(const double *d_x, uint64_t n, double *h_out, prodm_stream_t stream);
+
+/* ---- NegaBinaryBPEncoder<T, S>::encode / progressive_decode
+ *      (include/MDR/BitplaneEncoder/NegaBinaryBPEncoder.hpp:18-62,132-192).
+ *      Plane p (p = 0 is the most significant) is stored at
+ *      d_planes + p * plane_stride_words and holds ceil(n / B) words
+ *      (B = 32 for f32/u32, 64 for f64/u64); for u32 words the stride
+ *      must be even and >= ceil(n / 32) rounded up to even.
+ *      Decode: h_plane_ptrs[i] (host array of k device pointers) is plane
+ *      start + i; k == 0 writes zeros.                                    */
+int prodm_negabinary_encode_f32(const float *d_x, uint32_t n, int32_t exp, int num_bitplanes, uint32_t *d_planes,
+                                uint64_t plane_stride_words, prodm_stream_t stream);
+int prodm_negabinary_encode_f64(const double *d_x, uint32_t n, int32_t exp, int num_bitplanes, uint64_t *d_planes,
+                                uint64_t plane_stride_words, prodm_stream_t stream);
+int prodm_negabinary_decode_f32(const uint32_t *const *h_plane_ptrs, uint32_t n, int32_t exp, int start, int k,
+                                float *d_out, prodm_stream_t stream);
+int prodm_negabinary_decode_f64(const uint64_t *const *h_plane_ptrs, uint32_t n, int32_t exp, int start, int k,
+                                double *d_out, prodm_stream_t stream);
+
+/* ---- GroupedBPEncoder<T, S>::encode / progressive_decode
+ *      (include/MDR/BitplaneEncoder/GroupedBPEncoder.hpp:18-73,210-297).
+ *      Encode writes the per-block recording planes d_rb[nblocks], the
+ *      compacted words of plane p at d_planes + p * plane_stride_words
+ *      (stride >= 2 * nblocks) and d_words[p] = words in plane p. The
+ *      reference stream p is those words; stream 0 is prefixed with
+ *      u32 nblocks and the nblocks rb bytes. compat != 0 reproduces the
+ *      x86 result of (int64_t) on |v| >= 2^63 (nb = 64 only).
+ *      Decode takes k device pointers to reference-format streams
+ *      (stream 0 carries the header when start == 0) and keeps the
+ *      per-level state (rb, signs) in `state`, keyed by `level`.         */
+size_t prodm_grouped_workspace_bytes(uint32_t n, int num_bitplanes, int dtype);
+int prodm_grouped_encode_f32(const float *d_x, uint32_t n, int32_t exp, int num_bitplanes, int compat, uint8_t *d_rb,
+                             uint32_t *d_planes, uint64_t plane_stride_words, uint32_t *d_words, void *d_ws,
+                             size_t ws_bytes, prodm_stream_t stream);
+int prodm_grouped_encode_f64(const double *d_x, uint32_t n, int32_t exp, int num_bitplanes, int compat, uint8_t *d_rb,
+                             uint64_t *d_planes, uint64_t plane_stride_words, uint32_t *d_words, void *d_ws,
+                             size_t ws_bytes, prodm_stream_t stream);
+typedef struct prodm_grouped_state prodm_grouped_state;
+int prodm_grouped_state_create(prodm_grouped_state **out, int max_levels, int dtype);
+int prodm_grouped_state_destroy(prodm_grouped_state *state);
+int prodm_grouped_decode_f32(prodm_grouped_state *state, int level, const uint8_t *const *h_stream_ptrs, uint32_t n,
+                             int32_t exp, int start, int k, float *d_out, prodm_stream_t stream);
+int prodm_grouped_decode_f64(prodm_grouped_state *state, int level, const uint8_t *const *h_stream_ptrs, uint32_t n,
+                             int32_t exp, int start, int k, double *d_out, prodm_stream_t stream);
+
+/* ---- MaxErrorCollector<T>::collect_level_error
+ *      (include/MDR/ErrorCollector/MaxErrorCollector.hpp:14-25); host,
+ *      closed form, h_out[0..nb].                                        */
+void prodm_collect_max_f32(float max_level_error, int num_bitplanes, double *h_out);
+void prodm_collect_max_f64(double max_level_error, int num_bitplanes, double *h_out);
+
+/* ---- SquaredErrorCollector<T>::collect_level_error
+ *      (include/MDR/ErrorCollector/SquaredErrorCollector.hpp:23-65).
+ *      Each table entry is the exactly rounded sum of the reference's
+ *      per-element terms. Synchronizes the stream; h_out[0..nb].          */
+size_t prodm_collect_squared_workspace_bytes(uint64_t n, int num_bitplanes, int dtype);
+int prodm_collect_squared_f32(const float *d_x, uint64_t n, int num_bitplanes, float max_level_error, double *h_out,
+                              void *d_ws, size_t ws_bytes, prodm_stream_t stream);
+int prodm_collect_squared_f64(const double *d_x, uint64_t n, int num_bitplanes, double max_level_error, double *h_out,
+                              void *d_ws, size_t ws_bytes, prodm_stream_t stream);
+
+/* ---- device-resident refactor / retrieve pipeline: the GPU stages of
+ *      ComposedRefactor::refactor (ComposedRefactor.hpp:78-141: decompose,
+ *      interleave, max-abs, encode) and of ComposedReconstructor::reconstruct
+ *      (ComposedReconstructor.hpp:349-420: decode, reposition, recompose).
+ *      Interleave is fused into the decomposition (coefficients are written
+ *      straight into per-level buffers) and reposition into the
+ *      recomposition. Lossless compression and file I/O stay on the host.  */
+typedef struct prodm_mdr_plan prodm_mdr_plan;
+int prodm_mdr_plan_create(prodm_mdr_plan **out, const uint32_t *dims, int ndims, int target_level, int num_bitplanes,
+                          int dtype, int encoder);
+int prodm_mdr_plan_destroy(prodm_mdr_plan *plan);
+/* device bytes held by the plan */
+uint64_t prodm_mdr_plan_device_bytes(const prodm_mdr_plan *plan);
+int prodm_mdr_refactor(prodm_mdr_plan *plan, const void *d_in, prodm_stream_t stream);
+/* level results (synchronizes): max |coef|, exponent, element count      */
+int prodm_mdr_level_info(prodm_mdr_plan *plan, int level, double *max_abs, int32_t *exp, uint64_t *elements);
+/* device pointer and byte size of plane p of level l (reference stream
+ * bytes without the GroupedBP stream-0 header); synchronizes for GroupedBP */
+int prodm_mdr_plane(prodm_mdr_plan *plan, int level, int plane, const void **d_ptr, uint64_t *bytes);
+/* GroupedBP: device pointer to the rb bytes of a level                   */
+int prodm_mdr_level_rb(prodm_mdr_plan *plan, int level, const uint8_t **d_rb, uint64_t *nblocks);
+/* device pointer to the interleaved coefficients of a level              */
+int prodm_mdr_level_coefficients(prodm_mdr_plan *plan, int level, void **d_ptr);
+/* Retrieve planes [0, planes[l]) of every level from the plan's plane
+ * storage and recompose to the full grid (d_out dense).                   */
+int prodm_mdr_retrieve(prodm_mdr_plan *plan, const uint8_t *planes_per_level, void *d_out, prodm_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

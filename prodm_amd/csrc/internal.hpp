@@ -1,0 +1,80 @@
+// internal.hpp -- launchers shared between the HIP translation units and the
+// C ABI layer. All launchers are asynchronous on `stream` and return a Status.
+#pragma once
+
+#include "common.hpp"
+
+namespace prodm {
+
+struct PlanePtrs {
+    const void *p[64];
+};
+
+// ---- fills (graph-safe replacements for hipMemsetAsync) --------------------
+int launch_fill_u32(uint32_t *dst, uint64_t count, uint32_t value, hipStream_t s);
+int launch_fill_u64(uint64_t *dst, uint64_t count, uint64_t value, hipStream_t s);
+
+// ---- max-abs (RefactorUtils.hpp:68-76) into an order key (atomicMax) --------
+template <class T> using KeyOf = typename std::conditional<sizeof(T) == 4, uint32_t, unsigned long long>::type;
+template <class T> int launch_maxabs(const T *x, uint64_t n, KeyOf<T> *key, hipStream_t s);
+
+// ---- NegaBinary (NegaBinaryBPEncoder.hpp:18-62, 132-192) ---------------------
+// planes: plane p at planes + p * stride (stride even for u32 words, and
+// >= nwords rounded up to even). Exponent from `exp` or, if key != nullptr,
+// from frexp(value(*key)) on the device.
+template <class T>
+int launch_nb_encode(const T *x, uint32_t n, int32_t exp, const KeyOf<T> *key, int nb,
+                     typename Traits<T>::Stream *planes, uint64_t stride, hipStream_t s);
+template <class T>
+int launch_nb_decode(const PlanePtrs &planes, uint32_t n, int32_t exp, const KeyOf<T> *key, int start, int k,
+                     T *out, hipStream_t s);
+
+// ---- GroupedBP (GroupedBPEncoder.hpp:18-73, 210-297) --------------------------
+template <class T> size_t gb_workspace_bytes(uint32_t n, int nb);
+// Writes rb[nblocks], compacted plane words (plane p at planes + p * stride,
+// capacity 2 * nblocks words each) and words_per_plane[nb] (device).
+template <class T>
+int launch_gb_encode(const T *x, uint32_t n, int32_t exp, const KeyOf<T> *key, int nb, int compat, uint8_t *rb,
+                     typename Traits<T>::Stream *planes, uint64_t stride, uint32_t *words_per_plane, void *ws,
+                     size_t ws_bytes, hipStream_t s);
+// streams: k device pointers in reference byte format (stream 0 carries the
+// 4 + nblocks byte header when start == 0). rb / signs: per-level state.
+template <class T>
+int launch_gb_decode(const PlanePtrs &streams, uint32_t n, int32_t exp, const KeyOf<T> *key, int start, int k,
+                     const uint8_t *rb, typename Traits<T>::Stream *signs, T *out, void *ws, size_t ws_bytes,
+                     uint64_t hdr0_bytes, hipStream_t s);
+template <class T> size_t gb_decode_workspace_bytes(uint32_t n, int start, int k);
+
+// ---- interleave (DirectInterleaver.hpp:12-63) --------------------------------
+template <class T>
+int launch_interleave(const T *data, const uint64_t *strides3, const uint32_t *fine3, const uint32_t *coarse3, T *buf,
+                      bool reposition, hipStream_t s);
+
+// ---- squared error collector (SquaredErrorCollector.hpp:23-65) ---------------
+template <class T> size_t sq_workspace_bytes(uint64_t n, int nb);
+template <class T>
+int launch_collect_squared(const T *x, uint64_t n, int nb, T max_level_error, void *ws, size_t ws_bytes,
+                           hipStream_t s);
+// finishes on the host after the stream is synchronized: exact rounding
+template <class T> int finish_collect_squared(const void *h_ws_copy, int nb, T max_level_error, double *out);
+template <class T> size_t sq_result_bytes(int nb);
+
+// ---- MGARD ------------------------------------------------------------------
+size_t mgard_workspace_bytes(const LevelGeom &g, size_t elem);
+// Coefficient addressing of a decompose/recompose: full array (reordered
+// positions with strides) or per-level interleaved buffers.
+template <class T> struct CoefSink {
+    T *full = nullptr;           // full-array mode
+    uint64_t s[3] = {0, 0, 1};   // strides of the full array
+    T *level[33] = {nullptr};    // interleaved mode: level buffers (0..L)
+};
+template <class T>
+int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g, const CoefSink<T> &sink,
+                    KeyOf<T> *level_keys, void *ws, size_t ws_bytes, hipStream_t s);
+template <class T>
+int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int unused, T *out, const uint64_t *out_strides,
+                    void *ws, size_t ws_bytes, hipStream_t s);
+int mgard_recompose_adjoint(const double *g_in, const uint64_t *in_strides, const LevelGeom &g, double *out,
+                            const uint64_t *out_strides, void *ws, size_t ws_bytes, hipStream_t s);
+
+}  // namespace prodm
