@@ -1,0 +1,283 @@
+#!/usr/bin/env python3
+"""Refactor + retrieve throughput of the MDR hot path on one MI355X.
+
+Workload (one "step"): refactor a 513^3 float32 field (6 levels, 32 bitplanes,
+NegaBinary encoder) and retrieve it at relative L-inf 1e-4 (planes per level
+chosen by MaxErrorEstimatorOB(3) + greedy interpreter from the level error
+tables), inputs resident in HBM. GPU stages only: lossless compression and
+file I/O stay on the host and are outside the timed region.
+
+value = 2 * input bytes / (t_refactor + t_retrieve)  [GB/s]
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import heapq
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_field_gpu(n, seed=0, nbumps=16, noise=1e-4):
+    """sin(2pi 3x) cos(2pi 5y) sin(2pi 2z) + Gaussian bumps + N(0, noise) on
+    the unit cube (x along the slowest dim); numpy PCG64 draws A, w, c, noise
+    in that order, evaluated in fp64 on the GPU, returned as float32."""
+    import torch
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(0.5, 2.0, nbumps)
+    w = rng.uniform(0.02, 0.1, nbumps)
+    c = rng.uniform(0.0, 1.0, (nbumps, 3))
+    noise_h = rng.normal(0.0, noise, n * n * n)
+    dev = torch.device("cuda")
+    t = torch.linspace(0.0, 1.0, n, dtype=torch.float64, device=dev)
+    x, y, z = t.view(n, 1, 1), t.view(1, n, 1), t.view(1, 1, n)
+    u = torch.sin(2 * math.pi * 3 * x) * torch.cos(2 * math.pi * 5 * y) * torch.sin(2 * math.pi * 2 * z)
+    for b in range(nbumps):
+        gx = torch.exp(-((x - c[b, 0]) ** 2) / (2 * w[b] ** 2))
+        gy = torch.exp(-((y - c[b, 1]) ** 2) / (2 * w[b] ** 2))
+        gz = torch.exp(-((z - c[b, 2]) ** 2) / (2 * w[b] ** 2))
+        u = u + A[b] * (gx * gy * gz)
+    u = u + torch.from_numpy(noise_h).to(dev).view(n, n, n)
+    return u.to(torch.float32).contiguous().view(-1)
+
+
+def greedy_planes(level_bounds, level_plane_bytes, nb, tol, c):
+    """GreedyBasedSizeInterpreter::interpret_retrieve_size
+    (GreedyBasedSizeInterpreter.hpp:168-204) over MaxErrorCollector tables
+    (MaxErrorCollector.hpp:14-25) with MaxErrorEstimatorOB (float estimator,
+    MaxErrorEstimator.hpp:13-40). Returns planes per level."""
+    f32 = np.float32
+    L1 = len(level_bounds)
+    tables = []
+    for b in level_bounds:
+        _, e = math.frexp(b)
+        t = [float(b)] + [math.ldexp(1.0, e - k) for k in range(1, nb + 1)]
+        tables.append(t)
+    est = lambda err: float(f32(c) * f32(err))
+    idx = [0] * L1
+    acc = 0.0
+    for i in range(L1):
+        acc += est(tables[i][0])
+    heap = []
+    for i in range(L1):
+        gain = float(f32(c) * (f32(tables[i][0]) - f32(tables[i][1])))
+        heapq.heappush(heap, (-(gain / level_plane_bytes[i]), i))
+    met = False
+    while not met and heap:
+        _, i = heapq.heappop(heap)
+        j = idx[i]
+        acc -= est(tables[i][j])
+        acc += est(tables[i][j + 1])
+        if acc < tol:
+            met = True
+        idx[i] += 1
+        if idx[i] != nb:
+            gain = float(f32(c) * (f32(tables[i][idx[i]]) - f32(tables[i][idx[i] + 1])))
+            heapq.heappush(heap, (-(gain / level_plane_bytes[i]), i))
+    return idx, acc
+
+
+def cpu_baseline(u_host, dims, L, nb, planes):
+    """Single-threaded CPU restatement (oracle) of the same refactor +
+    retrieve on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+    t0 = time.perf_counter()
+    dec = ob.decompose(u_host, dims, L)
+    ld = ob.level_dims(dims, L)
+    streams, exps, bufs = [], [], []
+    for l in range(L + 1):
+        coarse = ld[l - 1] if l else np.zeros(3, np.uint32)
+        buf = ob.interleave(dec, dims, ld[l], coarse)
+        e = ob.frexp_exp(ob.maxabs(buf), np.float32)
+        streams.append(ob.nb_encode(buf, e, nb))
+        exps.append(e)
+        bufs.append(buf.size)
+    t1 = time.perf_counter()
+    rec = np.zeros_like(u_host)
+    for l in range(L + 1):
+        coarse = ld[l - 1] if l else np.zeros(3, np.uint32)
+        vals = ob.nb_decode(streams[l], bufs[l], exps[l], 0, planes[l], np.float32)
+        ob.reposition(vals, rec, dims, ld[l], coarse)
+    rec = ob.recompose(rec, dims, L)
+    t2 = time.perf_counter()
+    return t1 - t0, t2 - t1, rec
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=513)
+    ap.add_argument("--levels", type=int, default=6)
+    ap.add_argument("--bitplanes", type=int, default=32)
+    ap.add_argument("--tol", type=float, default=1e-4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU layout is described in "
+              "DESIGN.md and is not run.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from prodm_amd import capi, mdr
+
+    n, L, nb = args.size, args.levels, args.bitplanes
+    dims = [n, n, n]
+    N = n ** 3
+    in_bytes = 4 * N
+    log(f"[bench] device {torch.cuda.get_device_name(0)}; field {n}^3 f32, L={L}, nb={nb}")
+    u = make_field_gpu(n)
+    vmin, vmax = torch.aminmax(u)
+    vrange = float(vmax - vmin)
+    tau = args.tol * vrange
+    plan = mdr.Plan(dims, L, nb, torch.float32, capi.NEGABINARY)
+    out = torch.empty_like(u)
+    log(f"[bench] plan holds {plan.device_bytes / 2**30:.2f} GiB on the device")
+
+    # retrieval plan from the level error bounds (4 * max for negabinary)
+    plan.refactor(u)
+    torch.cuda.synchronize()
+    bounds, plane_bytes = [], []
+    for l in range(L + 1):
+        m, e, el = plan.level_info(l)
+        bounds.append(float(np.float32(m) * np.float32(4.0)))
+        plane_bytes.append(((el + 31) // 32) * 4)
+    c_ob = float(np.float32((1.0 + 21.0 * math.sqrt(3) / 8) * 4))
+    planes, est = greedy_planes(bounds, plane_bytes, nb, tau, c_ob)
+    log(f"[bench] tau={tau:.4e} planes per level {planes} (estimated error {est:.4e})")
+    plan.retrieve(planes, out)
+    torch.cuda.synchronize()
+    err = float(torch.max(torch.abs(out.double() - u.double())))
+    log(f"[bench] retrieval max error {err:.4e} (bound {tau:.4e})")
+    if not err <= tau:
+        log("[bench] FAIL: requested tolerance not met")
+        sys.exit(1)
+    retrieved_bytes = sum(planes[l] * plane_bytes[l] for l in range(L + 1))
+
+    # warmup
+    for _ in range(args.warmup):
+        plan.refactor(u)
+        plan.retrieve(planes, out)
+    torch.cuda.synchronize()
+
+    # timed region: K steps of refactor + retrieve, stage events on
+    s = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    plan.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ev[k][0].record(s)
+        plan.refactor(u)
+        ev[k][1].record(s)
+        plan.retrieve(planes, out)
+        ev[k][2].record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    stages = plan.stage_times()
+    plan.profile(False)
+    t_ref = sum(a.elapsed_time(b) for a, b, _ in ev) / args.steps
+    t_ret = sum(b.elapsed_time(c) for _, b, c in ev) / args.steps
+    ms_step = wall * 1e3 / args.steps
+    gbs_ref = in_bytes / (t_ref * 1e6)
+    gbs_ret = in_bytes / (t_ret * 1e6)
+    value = 2 * in_bytes / ((t_ref + t_ret) * 1e6)
+    log(f"[bench] refactor {t_ref:.3f} ms ({gbs_ref:.1f} GB/s), retrieve {t_ret:.3f} ms ({gbs_ret:.1f} GB/s), "
+        f"wall {ms_step:.3f} ms/step")
+
+    # algorithmic bytes (refactor: decompose 2 s n_l per level + encoder read s N
+    # + planes written; retrieve: planes read + s N written + recompose 2 s n_l)
+    ld, el, _ = mdr.level_geometry(dims, L)
+    pts = [int(np.prod(ld[l].astype(np.int64))) for l in range(L + 1)]
+    P = sum(nb * plane_bytes[l] for l in range(L + 1))
+    b_ref = sum(2 * 4 * pts[l] for l in range(1, L + 1)) + 4 * N + P
+    b_ret = retrieved_bytes + 4 * N + sum(2 * 4 * pts[l] for l in range(1, L + 1))
+    # per-kernel roofline: algorithmic bytes of one launch at the finest level
+    eltop = int(el[L])
+    kern = {
+        "decompose_load_top": ("k_load<float,decompose>", 2 * 4 * pts[L]),
+        "encode_top": ("k_nb_encode<float,32>", 4 * eltop + nb * plane_bytes[L]),
+        "decode_top": ("k_nb_decode<float>", planes[L] * plane_bytes[L] + 4 * eltop),
+        "recompose_load_top": ("k_load<float,recompose>", 4 * eltop),
+        "recompose_interp_top": ("k_interp<float>", 4 * eltop + 4 * pts[L - 1] + 4 * pts[L]),
+    }
+    stage_ms = {k: v[0] / max(v[1], 1) for k, v in stages.items()}
+    dom = max(kern, key=lambda k: stage_ms.get(k, 0.0))
+    dname, dbytes = kern[dom]
+    dms = stage_ms[dom]
+    achieved = dbytes / (dms * 1e6)
+    roof = {"bound": "hbm", "kernel": dname, "stage": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+            "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+            "algorithmic_bytes_per_launch": dbytes, "avg_launch_ms": round(dms, 5)}
+    per_kernel = {k: {"avg_ms": round(stage_ms[k], 5), "algorithmic_GBps": round(v[1] / (stage_ms[k] * 1e6), 1)}
+                  for k, v in kern.items() if stage_ms.get(k, 0) > 0}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline: single-threaded CPU restatement on the host, same field ...")
+        u_host = u.cpu().numpy()
+        tr, tt, rec = cpu_baseline(u_host, dims, L, nb, planes)
+        cerr = float(np.max(np.abs(rec.astype(np.float64) - u_host)))
+        cpu_gbs = 2 * in_bytes / ((tr + tt) * 1e9)
+        cpu = {"value": round(cpu_gbs, 4), "unit": "GB/s", "cores": 1, "kind": "port",
+               "sample": f"the full {n}^3 field, one refactor + one retrieve with the same planes per level "
+                         f"(refactor {tr:.2f} s, retrieve {tt:.2f} s, max error {cerr:.3e})"}
+        log(f"[bench] CPU: refactor {tr:.2f} s, retrieve {tt:.2f} s -> {cpu_gbs:.3f} GB/s")
+
+    res = {
+        "metric": "refactor+retrieve GB/s at 513^3 fp32, rel Linf 1e-4",
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"MDR refactor + retrieve of a {n}^3 float32 field (sin*cos*sin + 16 Gaussian bumps + "
+                        f"N(0,1e-4), seed 0), {L} levels, {nb} bitplanes, NegaBinary encoder, retrieval at "
+                        f"relative L-inf {args.tol:g} (MaxErrorEstimatorOB + greedy interpreter)",
+            "dims": dims, "levels": L, "bitplanes": nb, "encoder": "negabinary", "tolerance_rel_linf": args.tol,
+        },
+        "refactor_ms": round(t_ref, 4),
+        "retrieve_ms": round(t_ret, 4),
+        "refactor_GBps": round(gbs_ref, 2),
+        "retrieve_GBps": round(gbs_ret, 2),
+        "refactor_hbm_frac": round(b_ref / (t_ref * 1e6) / PEAK_HBM_GBS, 4),
+        "retrieve_hbm_frac": round(b_ret / (t_ret * 1e6) / PEAK_HBM_GBS, 4),
+        "algorithmic_bytes": {"refactor": b_ref, "retrieve": b_ret},
+        "planes_per_level": planes,
+        "retrieved_plane_bytes": retrieved_bytes,
+        "max_abs_error": err,
+        "tolerance_abs": tau,
+        "stages_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in stages.items() if v[1]},
+        "kernels": per_kernel,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+    plan.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -161,6 +161,18 @@ int prodm_mdr_level_coefficients(prodm_mdr_plan *plan, int level, void **d_ptr);
 /* Retrieve planes [0, planes[l]) of every level from the plan's plane
  * storage and recompose to the full grid (d_out dense).                   */
 int prodm_mdr_retrieve(prodm_mdr_plan *plan, const uint8_t *planes_per_level, void *d_out, prodm_stream_t stream);
+/* Per-stage timing with HIP events recorded on the launch stream around
+ * each kernel group (no counterpart in ProDM, whose Timer wraps whole
+ * calls, RefactorUtils.hpp:145-170). Stages: 0 decompose load (finest
+ * level), 1 load (other levels), 2 line solves (finest), 3 solves (other),
+ * 4 coarsest copy, 5 encode (finest), 6 encode (other), 7 decode (finest),
+ * 8 decode (other), 9 recompose load (finest), 10 (other), 11 recompose
+ * solves (finest), 12 (other), 13 interpolation (finest), 14 (other).
+ * stage_times synchronizes, returns summed ms and launch-group counts since
+ * the previous call, and resets.                                           */
+#define PRODM_NUM_STAGES 15
+int prodm_mdr_profile(prodm_mdr_plan *plan, int enable);
+int prodm_mdr_stage_times(prodm_mdr_plan *plan, double *ms, uint32_t *counts, int nstages);
 
 #ifdef __cplusplus
 }

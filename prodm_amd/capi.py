@@ -58,7 +58,15 @@ _SIGS = {
     "prodm_mdr_level_rb": (C.c_int, [vp, C.c_int, vp, vp]),
     "prodm_mdr_level_coefficients": (C.c_int, [vp, C.c_int, vp]),
     "prodm_mdr_retrieve": (C.c_int, [vp, vp, vp, vp]),
+    "prodm_mdr_profile": (C.c_int, [vp, C.c_int]),
+    "prodm_mdr_stage_times": (C.c_int, [vp, vp, vp, C.c_int]),
 }
+
+
+STAGES = ["decompose_load_top", "decompose_load_rest", "decompose_solve_top", "decompose_solve_rest",
+          "decompose_copy", "encode_top", "encode_rest", "decode_top", "decode_rest", "recompose_load_top",
+          "recompose_load_rest", "recompose_solve_top", "recompose_solve_rest", "recompose_interp_top",
+          "recompose_interp_rest"]
 
 
 class ProdmError(RuntimeError):

@@ -12,6 +12,41 @@ static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
 const std::string &last_error() { return g_last_error; }
 
+void StageTimer::begin(int stage) {
+    if (used + 2 > pool.size()) {
+        const size_t grow = std::max<size_t>(64, pool.size());
+        for (size_t i = 0; i < grow; i++) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) { on = false; return; }
+            pool.push_back(e);
+        }
+    }
+    (void)hipEventRecord(pool[used], stream);
+    marks.emplace_back(stage, used);
+    used += 2;
+    open_stage = stage;
+}
+void StageTimer::end() {
+    if (open_stage < 0 || marks.empty()) return;
+    (void)hipEventRecord(pool[marks.back().second + 1], stream);
+    open_stage = -1;
+}
+int StageTimer::collect(double *ms, uint32_t *counts, int n) {
+    for (int i = 0; i < n; i++) { ms[i] = 0; counts[i] = 0; }
+    for (auto &m : marks) {
+        float t = 0;
+        if (hipEventSynchronize(pool[m.second + 1]) != hipSuccess) return kErrHip;
+        if (hipEventElapsedTime(&t, pool[m.second], pool[m.second + 1]) != hipSuccess) return kErrHip;
+        if (m.first < n) { ms[m.first] += t; counts[m.first]++; }
+    }
+    marks.clear();
+    used = 0;
+    return kOk;
+}
+StageTimer::~StageTimer() {
+    for (auto e : pool) (void)hipEventDestroy(e);
+}
+
 LevelGeom make_geom(const uint32_t *dims, int nd, int L) {
     LevelGeom g;
     g.nd = nd;
@@ -226,6 +261,7 @@ struct prodm_mdr_plan {
     void *gbws = nullptr;
     size_t gbws_bytes = 0;
     uint64_t device_bytes = 0;
+    prodm::StageTimer timer;
 };
 
 namespace prodm {
@@ -255,9 +291,12 @@ template <class T> static int plan_refactor(prodm_mdr_plan *p, const T *in, hipS
     CoefSink<T> sink;
     for (int l = 0; l <= g.L; l++) sink.level[l] = static_cast<T *>(p->levels) + p->level_off[l];
     const uint64_t st[3] = {(uint64_t)g.dims[1] * g.dims[2], g.dims[2], 1};
-    rc = mgard_decompose<T>(in, st, g, sink, keys, p->ws, p->ws_bytes, s);
+    StageTimer *tm = p->timer.on ? &p->timer : nullptr;
+    if (tm) tm->stream = s;
+    rc = mgard_decompose<T>(in, st, g, sink, keys, p->ws, p->ws_bytes, s, tm);
     if (rc) return rc;
-    for (int l = 0; l <= g.L; l++) {
+    for (int l = g.L; l >= 0; l--) {
+        StageScope sc(tm, l == g.L ? kStEncTop : kStEncRest);
         S *planes = static_cast<S *>(p->planes) + p->plane_off[l];
         if (p->encoder == PRODM_ENC_NEGABINARY)
             rc = launch_nb_encode<T>(sink.level[l], (uint32_t)g.elems[l], 0, keys + l, p->nb, planes, p->plane_stride[l], s);
@@ -277,7 +316,10 @@ template <class T> static int plan_retrieve(prodm_mdr_plan *p, const uint8_t *k,
     CoefSink<T> src;
     for (int l = 0; l <= g.L; l++) src.level[l] = static_cast<T *>(p->levels) + p->level_off[l];
     int rc = kOk;
-    for (int l = 0; l <= g.L; l++) {
+    StageTimer *tm = p->timer.on ? &p->timer : nullptr;
+    if (tm) tm->stream = s;
+    for (int l = g.L; l >= 0; l--) {
+        StageScope sc(tm, l == g.L ? kStDecodeTop : kStDecodeRest);
         if (k[l] > p->nb) { set_error("retrieve: more planes than encoded"); return kErrArg; }
         S *planes = static_cast<S *>(p->planes) + p->plane_off[l];
         PlanePtrs pp{};
@@ -293,7 +335,7 @@ template <class T> static int plan_retrieve(prodm_mdr_plan *p, const uint8_t *k,
         if (rc) return rc;
     }
     const uint64_t st[3] = {(uint64_t)g.dims[1] * g.dims[2], g.dims[2], 1};
-    return mgard_recompose<T>(src, g, 0, out, st, p->ws, p->ws_bytes, s);
+    return mgard_recompose<T>(src, g, 0, out, st, p->ws, p->ws_bytes, s, tm);
 }
 
 }  // namespace prodm
@@ -567,6 +609,24 @@ int prodm_mdr_level_coefficients(prodm_mdr_plan *p, int level, void **d_ptr) {
     if (!p || level < 0 || level > p->g.L) { set_error("level_coefficients: bad level"); return kErrArg; }
     *d_ptr = static_cast<char *>(p->levels) + p->elem * p->level_off[level];
     return kOk;
+}
+
+int prodm_mdr_profile(prodm_mdr_plan *p, int enable) {
+    if (!p) { set_error("profile: null plan"); return kErrArg; }
+    double ms[kNumStages];
+    uint32_t cnt[kNumStages];
+    int rc = p->timer.collect(ms, cnt, kNumStages);  // drop pending marks
+    p->timer.on = enable != 0;
+    return rc;
+}
+
+int prodm_mdr_stage_times(prodm_mdr_plan *p, double *ms, uint32_t *counts, int nstages) {
+    if (!p || !ms || !counts || nstages < 1) { set_error("stage_times: bad arguments"); return kErrArg; }
+    std::vector<double> m(kNumStages);
+    std::vector<uint32_t> c(kNumStages);
+    int rc = p->timer.collect(m.data(), c.data(), kNumStages);
+    for (int i = 0; i < nstages; i++) { ms[i] = i < kNumStages ? m[i] : 0; counts[i] = i < kNumStages ? c[i] : 0; }
+    return rc;
 }
 
 int prodm_mdr_retrieve(prodm_mdr_plan *p, const uint8_t *k, void *d_out, prodm_stream_t stream) {

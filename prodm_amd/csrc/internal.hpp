@@ -2,12 +2,39 @@
 // C ABI layer. All launchers are asynchronous on `stream` and return a Status.
 #pragma once
 
+#include <utility>
+#include <vector>
+
 #include "common.hpp"
 
 namespace prodm {
 
 struct PlanePtrs {
     const void *p[64];
+};
+
+// ---- per-stage timing with HIP events on the launch stream -------------------
+enum Stage : int {
+    kStDecLoadTop = 0, kStDecLoadRest, kStDecSolveTop, kStDecSolveRest, kStDecCopy, kStEncTop, kStEncRest,
+    kStDecodeTop, kStDecodeRest, kStRecLoadTop, kStRecLoadRest, kStRecSolveTop, kStRecSolveRest, kStRecInterpTop,
+    kStRecInterpRest, kNumStages
+};
+struct StageTimer {
+    bool on = false;
+    hipStream_t stream = nullptr;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::vector<std::pair<int, size_t>> marks;  // (stage, index of the start event; end = +1)
+    int open_stage = -1;
+    void begin(int stage);
+    void end();
+    int collect(double *ms, uint32_t *counts, int n);  // synchronizes, then resets
+    ~StageTimer();
+};
+struct StageScope {
+    StageTimer *t;
+    StageScope(StageTimer *tm, int stage) : t(tm && tm->on ? tm : nullptr) { if (t) t->begin(stage); }
+    ~StageScope() { if (t) t->end(); }
 };
 
 // ---- fills (graph-safe replacements for hipMemsetAsync) --------------------
@@ -70,10 +97,10 @@ template <class T> struct CoefSink {
 };
 template <class T>
 int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g, const CoefSink<T> &sink,
-                    KeyOf<T> *level_keys, void *ws, size_t ws_bytes, hipStream_t s);
+                    KeyOf<T> *level_keys, void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm = nullptr);
 template <class T>
 int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int unused, T *out, const uint64_t *out_strides,
-                    void *ws, size_t ws_bytes, hipStream_t s);
+                    void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm = nullptr);
 int mgard_recompose_adjoint(const double *g_in, const uint64_t *in_strides, const LevelGeom &g, double *out,
                             const uint64_t *out_strides, void *ws, size_t ws_bytes, hipStream_t s);
 

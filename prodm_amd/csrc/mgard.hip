@@ -643,7 +643,7 @@ size_t mgard_workspace_bytes(const LevelGeom &g, size_t elem) { return ws_layout
 
 template <class T>
 int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g, const CoefSink<T> &sink,
-                    KeyOf<T> *level_keys, void *ws, size_t ws_bytes, hipStream_t s) {
+                    KeyOf<T> *level_keys, void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm) {
     const WsLayout w = ws_layout(g, sizeof(T));
     if (ws_bytes < w.total) { set_error("decompose: workspace too small"); return kErrArg; }
     char *base = static_cast<char *>(ws);
@@ -665,9 +665,16 @@ int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g,
         P.uc = uc;
         P.f = f;
         P.key = level_keys ? level_keys + l : nullptr;
-        int rc = launch_load<T, kDecompose>(P, ilv, s);
+        int rc;
+        {
+            StageScope sc(tm, l == g.L ? kStDecLoadTop : kStDecLoadRest);
+            rc = launch_load<T, kDecompose>(P, ilv, s);
+        }
         if (rc) return rc;
-        rc = solve_all<T>(g, l, f, tab, F, kEpiAdd, uc, uc, s);
+        {
+            StageScope sc(tm, l == g.L ? kStDecSolveTop : kStDecSolveRest);
+            rc = solve_all<T>(g, l, f, tab, F, kEpiAdd, uc, uc, s);
+        }
         if (rc) return rc;
         cur = uc;
         cs[0] = (uint64_t)g.ld[l - 1][1] * g.ld[l - 1][2];
@@ -675,6 +682,7 @@ int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g,
         which ^= 1;
     }
     // level 0 (coarsest grid) into the sink
+    StageScope sc0(tm, kStDecCopy);
     const uint32_t *c0 = g.ld[0];
     if (ilv) {
         if (g.L == 0) return copy_box<T>(in, in_strides[0], in_strides[1], sink.level[0], (uint64_t)c0[1] * c0[2], c0[2], c0, s);
@@ -691,13 +699,13 @@ int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g,
     return kOk;
 }
 template int mgard_decompose<float>(const float *, const uint64_t *, const LevelGeom &, const CoefSink<float> &,
-                                    uint32_t *, void *, size_t, hipStream_t);
+                                    uint32_t *, void *, size_t, hipStream_t, StageTimer *);
 template int mgard_decompose<double>(const double *, const uint64_t *, const LevelGeom &, const CoefSink<double> &,
-                                     unsigned long long *, void *, size_t, hipStream_t);
+                                     unsigned long long *, void *, size_t, hipStream_t, StageTimer *);
 
 template <class T>
 int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int /*unused*/, T *out, const uint64_t *out_strides,
-                    void *ws, size_t ws_bytes, hipStream_t s) {
+                    void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm) {
     const WsLayout w = ws_layout(g, sizeof(T));
     if (ws_bytes < w.total) { set_error("recompose: workspace too small"); return kErrArg; }
     char *base = static_cast<char *>(ws);
@@ -722,9 +730,15 @@ int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int /*unused*/, 
     for (int l = 1; l <= g.L; l++) {
         LoadParams<T> P = load_params(g, l, src);
         P.f = f;
-        rc = launch_load<T, kRecompose>(P, ilv, s);
+        {
+            StageScope sc(tm, l == g.L ? kStRecLoadTop : kStRecLoadRest);
+            rc = launch_load<T, kRecompose>(P, ilv, s);
+        }
         if (rc) return rc;
-        rc = solve_all<T>(g, l, f, tab, F, kEpiSub, cur, cur, s);  // u_c = u_c' - z, in place
+        {
+            StageScope sc(tm, l == g.L ? kStRecSolveTop : kStRecSolveRest);
+            rc = solve_all<T>(g, l, f, tab, F, kEpiSub, cur, cur, s);  // u_c = u_c' - z, in place
+        }
         if (rc) return rc;
         const uint32_t *n = g.ld[l];
         T *dst;
@@ -732,8 +746,11 @@ int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int /*unused*/, 
         if (l == g.L) { dst = out; d0 = out_strides[0]; d1 = out_strides[1]; }
         else { dst = bufs[which ^ 1]; d0 = (uint64_t)n[1] * n[2]; d1 = n[2]; }
         dim3 grid(div_up(n[2], 256), n[1], n[0]);
-        if (ilv) hipLaunchKernelGGL((k_interp<T, true>), grid, dim3(256), 0, s, P, cur, dst, d0, d1);
-        else hipLaunchKernelGGL((k_interp<T, false>), grid, dim3(256), 0, s, P, cur, dst, d0, d1);
+        {
+            StageScope sc(tm, l == g.L ? kStRecInterpTop : kStRecInterpRest);
+            if (ilv) hipLaunchKernelGGL((k_interp<T, true>), grid, dim3(256), 0, s, P, cur, dst, d0, d1);
+            else hipLaunchKernelGGL((k_interp<T, false>), grid, dim3(256), 0, s, P, cur, dst, d0, d1);
+        }
         PRODM_CHECK_LAUNCH();
         which ^= 1;
         cur = bufs[which];
@@ -741,8 +758,8 @@ int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int /*unused*/, 
     return kOk;
 }
 template int mgard_recompose<float>(const CoefSink<float> &, const LevelGeom &, int, float *, const uint64_t *, void *,
-                                    size_t, hipStream_t);
+                                    size_t, hipStream_t, StageTimer *);
 template int mgard_recompose<double>(const CoefSink<double> &, const LevelGeom &, int, double *, const uint64_t *,
-                                     void *, size_t, hipStream_t);
+                                     void *, size_t, hipStream_t, StageTimer *);
 
 }  // namespace prodm

@@ -239,9 +239,10 @@ class Plan:
     def level_rb(self, level):
         ptr, nblk = C.c_void_p(), C.c_uint64()
         check(lib().prodm_mdr_level_rb(self.h, level, C.byref(ptr), C.byref(nblk)), "level rb")
-        buf = torch.empty(int(nblk), dtype=torch.uint8, device="cuda")
+        nblk = int(nblk.value)
+        buf = torch.empty(nblk, dtype=torch.uint8, device="cuda")
         torch.cuda.synchronize()
-        _d2d(buf, ptr.value, int(nblk))
+        _d2d(buf, ptr.value, nblk)
         return buf.cpu().numpy()
 
     def coefficients(self, level):
@@ -259,6 +260,17 @@ class Plan:
         k = (C.c_uint8 * (self.L + 1))(*[int(v) for v in planes_per_level])
         check(lib().prodm_mdr_retrieve(self.h, k, _ptr(out), _stream()), "retrieve")
         return out
+
+    def profile(self, enable=True):
+        check(lib().prodm_mdr_profile(self.h, int(enable)), "profile")
+
+    def stage_times(self):
+        """{stage: (ms summed, launch groups)} since the previous call."""
+        n = len(capi.STAGES)
+        ms = (C.c_double * n)()
+        cnt = (C.c_uint32 * n)()
+        check(lib().prodm_mdr_stage_times(self.h, ms, cnt, n), "stage times")
+        return {capi.STAGES[i]: (ms[i], cnt[i]) for i in range(n)}
 
     def close(self):
         if getattr(self, "h", None):
