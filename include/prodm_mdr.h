@@ -170,6 +170,32 @@ int prodm_mdr_retrieve(prodm_mdr_plan *plan, const uint8_t *planes_per_level, vo
  * solves (finest), 12 (other), 13 interpolation (finest), 14 (other).
  * stage_times synchronizes, returns summed ms and launch-group counts since
  * the previous call, and resets.                                           */
+/* ---- retrieval planning (host only) --------------------------------------
+ * Size interpreters of include/MDR/SizeInterpreter/GreedyBasedSizeInterpreter.hpp:23-250
+ * and BasicSizeInterpreter.hpp:11-112 with the estimators of
+ * include/MDR/ErrorEstimator/MaxErrorEstimator.hpp:7-68 and
+ * SquaredErrorEstimator.hpp:7-68 (C++ in include/prodm/interpreters.hpp).
+ * Level l has counts[l] planes: sizes holds counts[l] plane byte sizes per
+ * level back to back, errors counts[l]+1 table entries per level. index[l]
+ * (planes already held) is updated; out_sizes[l] = bytes to read next.
+ * dtype selects the estimator's arithmetic (ProDM instantiates it with the
+ * field type). target_level = L of the S-norm / L2 weights (>= nlevels-1). */
+#define PRODM_PLAN_GREEDY 0
+#define PRODM_PLAN_SIGN_EXCLUDE_GREEDY 1
+#define PRODM_PLAN_NEGABINARY_GREEDY 2
+#define PRODM_PLAN_INORDER 3
+#define PRODM_PLAN_ROUND_ROBIN 4
+#define PRODM_EST_MAX_OB 0
+#define PRODM_EST_MAX_HB 1
+#define PRODM_EST_SNORM 2
+#define PRODM_EST_L2_HB 3
+int prodm_interpret_retrieve_size(int planner, int estimator, int dtype, int ndims, int target_level, double s,
+                                  int nlevels, const int *counts, const uint32_t *sizes, const double *errors,
+                                  double tolerance, uint8_t *index, uint32_t *out_sizes);
+/* c of MaxErrorEstimatorOB (MaxErrorEstimator.hpp:15-31), in the estimator's
+ * arithmetic; 0 for ndims outside 1..3.                                    */
+double prodm_max_estimator_ob_constant(int ndims, int dtype);
+
 #define PRODM_NUM_STAGES 15
 int prodm_mdr_profile(prodm_mdr_plan *plan, int enable);
 int prodm_mdr_stage_times(prodm_mdr_plan *plan, double *ms, uint32_t *counts, int nstages);

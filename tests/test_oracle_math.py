@@ -1,0 +1,348 @@
+"""Pins of the CPU oracle by what the mathematics fixes, independent of the
+oracle's own formulas:
+
+* MGARD orthogonal decomposition (the arithmetic ProDM takes from MGARDx,
+  Decomposer/MGARD.hpp:14-35, absent from the sources): after L steps the
+  coarsest nodal values are the L2 projection of the fine piecewise
+  multilinear field, x = (P^T M P)^-1 P^T M u, with P the assembled
+  prolongation and M the assembled fine mass matrix (dense FP64 solve);
+  coefficients are u - I u_c at the coefficient nodes; multilinear fields
+  give zero coefficients; recompose inverts decompose; the adjoint of
+  recompose equals the dense transpose of the assembled recompose matrix.
+* interleave/reposition: brute force over the box difference.
+* codecs: truncation identities, stream sizes from an independent count of
+  recording planes, hand-built single-bit blocks.
+* squared collector: exact rational sum of the per-element terms.
+"""
+import itertools
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+
+# ---------------------------------------------------------------- MGARD -----
+
+
+def prolong_1d(nf):
+    """Coarse -> fine linear interpolation on n_f = 2m+1 nodes (n_f = 1: identity)."""
+    if nf == 1:
+        return np.ones((1, 1))
+    nc = (nf + 1) // 2
+    P = np.zeros((nf, nc))
+    for j in range(nc):
+        P[2 * j, j] = 1.0
+    for i in range(nc - 1):
+        P[2 * i + 1, i] = P[2 * i + 1, i + 1] = 0.5
+    return P
+
+
+def mass_1d(n):
+    """Linear finite-element mass matrix, unit spacing (n = 1: [1])."""
+    if n == 1:
+        return np.ones((1, 1))
+    M = np.zeros((n, n))
+    for i in range(n - 1):
+        M[i, i] += 1 / 3
+        M[i + 1, i + 1] += 1 / 3
+        M[i, i + 1] += 1 / 6
+        M[i + 1, i] += 1 / 6
+    return M
+
+
+def kron_all(mats):
+    out = np.ones((1, 1))
+    for m in mats:
+        out = np.kron(out, m)
+    return out
+
+
+def pad3(dims):
+    return [1] * (3 - len(dims)) + list(dims)
+
+
+def l2_projection(u, dims, L):
+    """Nodal values on the level-0 grid of the L2 projection of u (dense)."""
+    ld = ob.level_dims(dims, L)
+    Ptot = np.eye(int(np.prod(dims)))
+    for l in range(L, 0, -1):
+        Pl = kron_all([prolong_1d(int(n)) for n in ld[l]])
+        Ptot = Ptot @ Pl
+    M = kron_all([mass_1d(int(n)) for n in dims])
+    A = Ptot.T @ M @ Ptot
+    return np.linalg.solve(A, Ptot.T @ (M @ u)), ld[0]
+
+
+CASES = [([9, 17, 5], 1), ([9, 17, 5], 2), ([17, 9], 3), ([33], 4), ([1, 9, 17], 2), ([5, 5, 9], 2)]
+
+
+@pytest.mark.parametrize("dims,L", CASES)
+def test_coarse_values_are_l2_projection(dims, L):
+    rng = np.random.default_rng(1)
+    u = rng.standard_normal(int(np.prod(dims)))
+    got = ob.decompose(u, dims, L).reshape(dims)
+    x, c = l2_projection(u, dims, L)
+    box = got[tuple(slice(0, int(n)) for n in c)].ravel()
+    np.testing.assert_allclose(box, x, rtol=0, atol=1e-12 * np.max(np.abs(x)))
+
+
+@pytest.mark.parametrize("dims", [[9, 17, 5], [17, 9], [33], [1, 9, 5]])
+def test_coefficients_are_interpolation_residuals(dims):
+    """One step: coefficient nodes hold u - I u_c, coarse nodes first along
+    each dim (DirectInterleaver.hpp:23-30 reads them as the box difference)."""
+    rng = np.random.default_rng(2)
+    u = rng.standard_normal(int(np.prod(dims)))
+    got = ob.decompose(u, dims, 1).reshape(dims)
+    P = kron_all([prolong_1d(n) for n in dims])
+    U = u.reshape(dims)
+    coarse_idx = [np.arange(0, n, 2) if n > 1 else np.arange(1) for n in dims]
+    uc = U[np.ix_(*coarse_idx)].ravel()
+    W = (u - P @ uc).reshape(dims)
+
+    def rpos(i, n):
+        if n == 1:
+            return 0
+        m = (n - 1) // 2
+        return i // 2 if i % 2 == 0 else m + 1 + i // 2
+
+    for idx in itertools.product(*[range(n) for n in dims]):
+        if all(i % 2 == 0 for i in idx):
+            continue
+        pos = tuple(rpos(i, n) for i, n in zip(idx, dims))
+        assert abs(got[pos] - W[idx]) <= 1e-13, (idx, got[pos], W[idx])
+
+
+@pytest.mark.parametrize("dims,L", [([17, 9, 33], 2), ([33, 17], 3), ([65], 5)])
+def test_multilinear_fields_have_zero_coefficients(dims, L):
+    axes = [np.linspace(0, 1, n) for n in dims]
+    g = np.meshgrid(*axes, indexing="ij")
+    u = 0.3 + np.zeros(dims)
+    for d in range(len(dims)):
+        u = u + (d + 1) * g[d]
+    if len(dims) > 1:
+        u = u + 2.5 * g[0] * g[1]
+    if len(dims) > 2:
+        u = u - 1.5 * g[0] * g[1] * g[2] + 0.7 * g[1] * g[2]
+    out = ob.decompose(u.ravel(), dims, L).reshape(dims)
+    ld = ob.level_dims(dims, L)
+    mask = np.ones(dims, bool)
+    mask[tuple(slice(0, int(n)) for n in ld[0])] = False
+    assert np.max(np.abs(out[mask])) <= 1e-12
+    # the level-0 grid holds the field's values at its nodes
+    step = [(n - 1) // (c - 1) if c > 1 else 1 for n, c in zip(dims, ld[0])]
+    want = u[tuple(slice(0, None, s) for s in step)]
+    np.testing.assert_allclose(out[tuple(slice(0, int(n)) for n in ld[0])], want, atol=1e-12)
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-14), (np.float32, 1e-6)])
+@pytest.mark.parametrize("dims,L", [([33, 17, 65], 3), ([129, 33], 4), ([257], 7), ([1, 33, 33], 4)])
+def test_recompose_inverts_decompose(dtype, tol, dims, L):
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal(int(np.prod(dims))).astype(dtype)
+    v = ob.recompose(ob.decompose(u, dims, L), dims, L)
+    assert np.max(np.abs(v.astype(np.float64) - u)) <= tol * np.max(np.abs(u))
+
+
+def test_strided_box_matches_dense_box():
+    """Decomposing a box inside a larger array (full strides) equals
+    decomposing the box alone, and leaves the rest of the array untouched."""
+    rng = np.random.default_rng(4)
+    full = [17, 9, 33]
+    u = rng.standard_normal(int(np.prod(full)))
+    box = [9, 5, 17]
+    strides = [9 * 33, 33, 1]
+    got = ob.decompose(u, box, 2, strides=strides).reshape(full)
+    U = u.reshape(full)
+    dense = ob.decompose(np.ascontiguousarray(U[:9, :5, :17]).ravel(), box, 2).reshape(box)
+    np.testing.assert_array_equal(got[:9, :5, :17], dense)
+    mask = np.ones(full, bool)
+    mask[:9, :5, :17] = False
+    np.testing.assert_array_equal(got[mask], U[mask])
+
+
+@pytest.mark.parametrize("dims,L", [([5, 9, 5], 2), ([9, 17], 3), ([17], 4)])
+def test_recompose_adjoint_is_dense_transpose(dims, L):
+    N = int(np.prod(dims))
+    A = np.zeros((N, N))
+    for j in range(N):
+        e = np.zeros(N)
+        e[j] = 1.0
+        A[:, j] = ob.recompose(e, dims, L)
+    rng = np.random.default_rng(5)
+    for _ in range(3):
+        g = rng.standard_normal(N)
+        np.testing.assert_allclose(ob.recompose_adjoint(g, dims, L), A.T @ g, rtol=0, atol=1e-12 * np.abs(g).sum())
+
+
+# ----------------------------------------------------------- geometry -------
+
+
+@pytest.mark.parametrize("dims,L", [([513, 513, 513], 6), ([17, 9, 33], 2), ([100, 37, 8], 2), ([65], 5)])
+def test_level_elements_box_count(dims, L):
+    ld = ob.level_dims(dims, L)
+    for l in range(1, L + 1):
+        for d in range(len(dims)):
+            assert ld[l - 1][d] == (ld[l][d] >> 1) + 1
+    el = ob.level_elements(dims, L)
+    prev = 0
+    for l in range(L + 1):
+        box = int(np.prod(ld[l].astype(np.int64)))
+        assert int(el[l]) == box - prev
+        prev = box
+
+
+# ----------------------------------------------------------- interleave -----
+
+
+@pytest.mark.parametrize("dims,L", [([9, 17, 5], 2), ([17, 33], 3), ([33], 4)])
+def test_interleave_brute_force(dims, L):
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal(int(np.prod(dims))).astype(np.float32)
+    X = x.reshape(dims)
+    ld = ob.level_dims(dims, L)
+    y = np.zeros_like(x)
+    total = 0
+    for l in range(L + 1):
+        coarse = ld[l - 1] if l else np.zeros(len(dims), np.uint32)
+        want = [X[idx] for idx in itertools.product(*[range(int(n)) for n in ld[l]])
+                if not all(i < int(c) for i, c in zip(idx, coarse))]
+        got = ob.interleave(x, dims, ld[l], coarse)
+        np.testing.assert_array_equal(got, np.array(want, np.float32))
+        ob.reposition(got, y, dims, ld[l], coarse)
+        total += got.size
+    assert total == x.size
+    np.testing.assert_array_equal(y, x)
+
+
+# ----------------------------------------------------------- max ---------
+
+
+def test_maxabs_and_exponent():
+    rng = np.random.default_rng(7)
+    for dt in (np.float32, np.float64):
+        x = (rng.standard_normal(1001) * 3).astype(dt)
+        m = ob.maxabs(x)
+        assert m == np.max(np.abs(x))
+        e = ob.frexp_exp(m, dt)
+        assert 2.0 ** (e - 1) <= m < 2.0 ** e
+        assert e == math.frexp(float(m))[1]
+        x[5] = np.nan  # NaN never wins the '>' comparison
+        assert ob.maxabs(x) == m
+
+
+def test_collect_max_closed_form():
+    for dt, nb in ((np.float32, 32), (np.float64, 64)):
+        for b in (1.0, 0.75, 3.0, 1e-20):
+            t = ob.collect_max(dt(b), nb, dt)
+            e = math.frexp(float(dt(b)))[1]
+            assert t[0] == float(dt(b))
+            for k in range(1, nb + 1):
+                assert t[k] == 2.0 ** (e - k)
+
+
+# ----------------------------------------------------------- codecs -------
+
+
+def codec_x(n, dt, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal(n) * np.exp2(rng.integers(-10, 1, n))
+    x[rng.random(n) < 0.1] = 0
+    return x.astype(dt)
+
+
+@pytest.mark.parametrize("dt,nb", [(np.float32, 32), (np.float32, 16), (np.float64, 64), (np.float64, 40)])
+@pytest.mark.parametrize("n", [1, 31, 32, 33, 64, 65, 1000])
+def test_negabinary_truncation_identity(dt, nb, n):
+    x = codec_x(n, dt, n)
+    e = ob.frexp_exp(ob.maxabs(x), dt) if np.any(x) else 0
+    planes = ob.nb_encode(x, e, nb)
+    # nb even: decoding all planes gives trunc(x 2^(nb-e-2)) 2^(e+2-nb)
+    q = np.trunc(np.ldexp(x.astype(np.float64), nb - e - 2)).astype(np.int64)
+    want = np.ldexp(q.astype(dt), e + 2 - nb).astype(dt)
+    got = ob.nb_decode(planes, n, e, 0, nb, dt)
+    np.testing.assert_array_equal(got, want)
+    # progressive deltas over a split add up to the one-shot result
+    a = nb // 3
+    d1 = ob.nb_decode(planes[:a], n, e, 0, a, dt).astype(np.float64)
+    d2 = ob.nb_decode(planes[a:], n, e, a, nb - a, dt).astype(np.float64)
+    np.testing.assert_allclose(d1 + d2, got, rtol=2.0 ** -20, atol=2.0 ** (e - nb + 3))
+
+
+def gb_rb(x, e, nb, B):
+    """Recording plane of each block: nb - bit length of the block's largest magnitude."""
+    mag = np.trunc(np.abs(np.ldexp(x.astype(np.float64), nb - e))).astype(np.uint64)
+    nblk = (x.size + B - 1) // B
+    rb = np.full(nblk, nb, np.int64)
+    for b in range(nblk):
+        m = int(mag[b * B:(b + 1) * B].max())
+        if m:
+            rb[b] = nb - m.bit_length()
+    return rb
+
+
+@pytest.mark.parametrize("dt,nb", [(np.float32, 32), (np.float32, 12), (np.float64, 60), (np.float64, 64)])
+@pytest.mark.parametrize("n", [1, 31, 32, 33, 64, 65, 1000])
+def test_grouped_sizes_and_truncation(dt, nb, n):
+    B = 32 if dt == np.float32 else 64
+    S = B // 8
+    x = codec_x(n, dt, 100 + n)
+    e = ob.frexp_exp(ob.maxabs(x), dt) if np.any(x) else 0
+    streams = ob.gb_encode(x, e, nb)
+    rb = gb_rb(x, e, nb, B)
+    nblk = rb.size
+    for p in range(nb):
+        want = S * (int(np.sum(rb <= p)) + int(np.sum(rb == p))) + (4 + nblk if p == 0 else 0)
+        assert len(streams[p]) == want, p
+    hdr = np.frombuffer(streams[0][:4], np.uint32)[0]
+    assert hdr == nblk
+    np.testing.assert_array_equal(np.frombuffer(streams[0][4:4 + nblk], np.uint8), rb.astype(np.uint8))
+    got = ob.GBDecoder(dt).decode(0, streams, n, e, 0, nb)
+    mag = np.trunc(np.abs(np.ldexp(x.astype(np.float64), nb - e)))
+    want = (np.sign(x) * np.ldexp(mag, e - nb)).astype(dt)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_single_bit_blocks(dt):
+    """One non-zero element per block, at every position: its plane word has
+    exactly that bit, the block's sign word is set for negatives."""
+    B = 32 if dt == np.float32 else 64
+    nb = 24
+    x = np.zeros(B * B, dt)
+    for b in range(B):
+        x[b * B + b] = (-1.0 if b % 2 else 1.0) * 0.5 ** (b % 7)
+    x[0] = 1.0
+    e = ob.frexp_exp(ob.maxabs(x), dt)
+    streams = ob.gb_encode(x, e, nb)
+    words = [np.frombuffer(s[(4 + B) if p == 0 else 0:], np.uint32 if B == 32 else np.uint64)
+             for p, s in enumerate(streams)]
+    pos = [0] * nb
+    for b in range(B):
+        k = b % 7  # magnitude 2^-k -> first set plane k + (nb - e) - nb ... with e = 1: plane k
+        rb = k
+        for p in range(rb, nb):
+            if p == rb:
+                sign = words[p][pos[p]]
+                pos[p] += 1
+                assert int(sign) == ((1 << b) if b % 2 else 0)
+            w = int(words[p][pos[p]])
+            pos[p] += 1
+            assert w == ((1 << b) if p == rb else 0), (b, p)
+    assert pos == [len(w) for w in words]
+
+
+# ----------------------------------------------------------- squared --------
+
+
+@pytest.mark.parametrize("nb", [32, 20])
+def test_squared_collector_is_exactly_rounded(nb):
+    """Each table entry of the exact mode is the correctly rounded sum of the
+    per-element terms (a term = the sequential table of that element alone)."""
+    x = codec_x(300, np.float32, 9)
+    m = np.float32(np.max(np.abs(x)))
+    terms = [ob.collect_squared(x[i:i + 1], nb, m, mode="seq") for i in range(x.size)]
+    want = [float(sum((Fraction(float(t[k])) for t in terms), Fraction(0))) for k in range(nb + 1)]
+    got = ob.collect_squared(x, nb, m, mode="exact")
+    np.testing.assert_array_equal(got, np.array(want))
