@@ -161,6 +161,22 @@ int prodm_mdr_level_coefficients(prodm_mdr_plan *plan, int level, void **d_ptr);
 /* Retrieve planes [0, planes[l]) of every level from the plan's plane
  * storage and recompose to the full grid (d_out dense).                   */
 int prodm_mdr_retrieve(prodm_mdr_plan *plan, const uint8_t *planes_per_level, void *d_out, prodm_stream_t stream);
+/* Reference-format byte stream of one plane, copied to host memory
+ * (NegaBinary: ceil(n/B) words, NegaBinaryBPEncoder.hpp:241-249; GroupedBP
+ * plane 0 carries the u32 nblocks + u8 rb[nblocks] prefix,
+ * GroupedBPEncoder.hpp:64-71). h_dst NULL queries *bytes only. Synchronizes
+ * the stream. These are the streams BitplaneEncoder::encode returns.     */
+int prodm_mdr_plane_stream(prodm_mdr_plan *plan, int level, int plane, uint8_t *h_dst, uint64_t capacity,
+                           uint64_t *bytes, prodm_stream_t stream);
+/* Inverse: loads one plane's stream (as written by the encoder) into the
+ * plan's plane storage, so that prodm_mdr_retrieve decodes stored planes
+ * (the decode side of ComposedReconstructor.hpp:236-253). Synchronizes.   */
+int prodm_mdr_load_plane_stream(prodm_mdr_plan *plan, int level, int plane, const uint8_t *h_src, uint64_t bytes,
+                                prodm_stream_t stream);
+/* Sets a level's max |coefficient| (whose exponent drives decoding) for
+ * planes loaded from storage: the level error bound of the metadata
+ * (divided by 4 for negabinary, ComposedReconstructor.hpp:202-204).       */
+int prodm_mdr_set_level_max(prodm_mdr_plan *plan, int level, double max_abs, prodm_stream_t stream);
 /* Per-stage timing with HIP events recorded on the launch stream around
  * each kernel group (no counterpart in ProDM, whose Timer wraps whole
  * calls, RefactorUtils.hpp:145-170). Stages: 0 decompose load (finest

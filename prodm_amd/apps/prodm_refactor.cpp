@@ -1,0 +1,101 @@
+// prodm_refactor -- refactor a raw binary field into ProDM's files on the GPU
+// (counterpart of ProDM's test/test_mdr_refactor.cpp:34-86).
+//
+//   prodm_refactor <data file> <target level> <num bitplanes> <num dims> <dims...>
+//                  [--f64] [--grouped] [--composed] [--squared] [--out DIR]
+//
+// Writes DIR/metadata.bin and DIR/level_<l>.bin (default DIR refactored_data).
+// Default: float, negabinary encoder, AdaptiveLevelCompressor(64), the
+// device-resident GpuRefactor; --composed drives the GPU components through
+// prodm::ComposedRefactor instead (same files).
+#include <sys/stat.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "prodm/prodm.hpp"
+
+using namespace prodm;
+
+template <class T>
+static std::vector<T> read_raw(const std::string &path, size_t n) {
+    std::vector<T> v(n);
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f || std::fread(v.data(), sizeof(T), n, f) != n) fatal("cannot read " + std::to_string(n) + " values from " + path);
+    std::fclose(f);
+    return v;
+}
+
+template <class T, class S>
+static int run(const std::string &file, const std::vector<uint32_t> &dims, int L, int nb, bool grouped, bool composed,
+               bool squared, const std::string &dir) {
+    const auto data = read_raw<T>(file, product(dims));
+    std::vector<std::string> files;
+    for (int l = 0; l <= L; l++) files.push_back(dir + "/level_" + std::to_string(l) + ".bin");
+    ConcatLevelFileWriter writer(dir + "/metadata.bin", files);
+    const auto t0 = std::chrono::steady_clock::now();
+    if (composed) {
+        Context ctx = make_context();
+        auto go = [&](auto encoder, auto collector) {
+            ComposedRefactor<T, GpuMGARDOrthogonalDecomposer<T>, GpuDirectInterleaver<T>, decltype(encoder),
+                             AdaptiveLevelCompressor, decltype(collector), ConcatLevelFileWriter>
+                r{GpuMGARDOrthogonalDecomposer<T>(ctx), GpuDirectInterleaver<T>(ctx), encoder,
+                  AdaptiveLevelCompressor(64), collector, writer};
+            r.negabinary = !grouped;
+            r.refactor(data.data(), dims, L, nb);
+        };
+        if (grouped) {
+            if (squared) go(GpuGroupedBPEncoder<T, S>(ctx), GpuSquaredErrorCollector<T>(ctx));
+            else go(GpuGroupedBPEncoder<T, S>(ctx), MaxErrorCollector<T>());
+        } else {
+            if (squared) go(GpuNegaBinaryBPEncoder<T, S>(ctx), GpuSquaredErrorCollector<T>(ctx));
+            else go(GpuNegaBinaryBPEncoder<T, S>(ctx), MaxErrorCollector<T>());
+        }
+    } else {
+        GpuRefactor<T, AdaptiveLevelCompressor, ConcatLevelFileWriter> r(
+            grouped ? PRODM_ENC_GROUPED : PRODM_ENC_NEGABINARY, AdaptiveLevelCompressor(64), writer, squared);
+        r.refactor(data.data(), dims, L, nb);
+    }
+    const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("Refactor time: %.6fs (%s, %s)\n", s, composed ? "composed GPU components" : "device-resident",
+                grouped ? "grouped" : "negabinary");
+    return 0;
+}
+
+int main(int argc, char **argv) {
+    if (argc < 6) {
+        std::fprintf(stderr, "usage: %s <file> <target level> <bitplanes> <ndims> <dims...> [--f64] [--grouped] "
+                             "[--composed] [--squared] [--out DIR]\n", argv[0]);
+        return 2;
+    }
+    int a = 1;
+    const std::string file = argv[a++];
+    const int L = std::atoi(argv[a++]);
+    int nb = std::atoi(argv[a++]);
+    const int nd = std::atoi(argv[a++]);
+    std::vector<uint32_t> dims;
+    for (int i = 0; i < nd && a < argc; i++) dims.push_back((uint32_t)std::atoi(argv[a++]));
+    bool f64 = false, grouped = false, composed = false, squared = false;
+    std::string dir = "refactored_data";
+    for (; a < argc; a++) {
+        const std::string o = argv[a];
+        if (o == "--f64") f64 = true;
+        else if (o == "--grouped") grouped = true;
+        else if (o == "--composed") composed = true;
+        else if (o == "--squared") squared = true;
+        else if (o == "--out" && a + 1 < argc) dir = argv[++a];
+        else fatal("unknown option " + o);
+    }
+    if (!grouped && (nb & 1)) {
+        nb += 1;  // negabinary needs an even plane count (test_mdr_refactor.cpp:40-43)
+        std::printf("Change to %d bitplanes for negabinary encoding\n", nb);
+    }
+    const int maxnb = f64 ? 64 : 32;
+    if (nb > maxnb) nb = maxnb;
+    mkdir(dir.c_str(), 0755);
+    return f64 ? run<double, uint64_t>(file, dims, L, nb, grouped, composed, squared, dir)
+               : run<float, uint32_t>(file, dims, L, nb, grouped, composed, squared, dir);
+}

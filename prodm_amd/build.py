@@ -45,6 +45,44 @@ def _compile(src):
     return obj, r.stderr
 
 
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+ZSTD_INC = os.environ.get("ZSTD_INC", "/opt/conda/include")  # zstd.h (libzstd.so.1 is the system library)
+BIN = os.path.join(HERE, "bin")
+HOST_LIB = os.path.join(HERE, "libprodm_host.so")
+# host C++ programs over the library: (output, source)
+PROGRAMS = [
+    ("prodm_refactor", os.path.join(HERE, "apps", "prodm_refactor.cpp")),
+    ("prodm_reconstruct", os.path.join(HERE, "apps", "prodm_reconstruct.cpp")),
+    ("test_cpp_pipeline", os.path.join(os.path.dirname(HERE), "tests", "cpp", "test_cpp_pipeline.cpp")),
+]
+
+
+def _cxx_host(out, src, shared=False, link_core=True):
+    cxx = os.environ.get("CXX", "g++")
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", f"-I{INCLUDE}", f"-I{ROCM}/include",
+           "-idirafter", ZSTD_INC, src, "-o", out]
+    if shared:
+        cmd[1:1] = ["-shared", "-fPIC"]
+    if link_core:
+        cmd += [f"-L{HERE}", "-lprodm_mdr", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/.."]
+    cmd += [f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib", "-l:libzstd.so.1", "-pthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{cxx} failed for {os.path.basename(src)}:\n{r.stderr}")
+    return out
+
+
+def build_host(jobs=None):
+    """Host C++ over the library: libprodm_host.so, the apps and the C++ tests."""
+    os.makedirs(BIN, exist_ok=True)
+    jobs = jobs or min(4, os.cpu_count() or 1)
+    tasks = [(HOST_LIB + ".tmp", os.path.join(CSRC, "host_capi.cpp"), True, False)]
+    tasks += [(os.path.join(BIN, name), src, False, True) for name, src in PROGRAMS]
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(lambda t: _cxx_host(*t), tasks))
+    os.replace(HOST_LIB + ".tmp", HOST_LIB)
+
+
 def build(jobs=None, verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     srcs = SOURCES + HOST_SOURCES
@@ -61,6 +99,7 @@ def build(jobs=None, verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_host(jobs)
     return LIB
 
 

@@ -629,6 +629,78 @@ int prodm_mdr_stage_times(prodm_mdr_plan *p, double *ms, uint32_t *counts, int n
     return rc;
 }
 
+int prodm_mdr_plane_stream(prodm_mdr_plan *p, int level, int plane, uint8_t *h_dst, uint64_t capacity,
+                           uint64_t *bytes, prodm_stream_t stream) {
+    if (!p || level < 0 || level > p->g.L || plane < 0 || plane >= p->nb || !bytes) {
+        set_error("plane_stream: bad arguments");
+        return kErrArg;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    PRODM_HIP(hipStreamSynchronize(s));
+    const void *src = nullptr;
+    uint64_t body = 0;
+    if (int rc = prodm_mdr_plane(p, level, plane, &src, &body)) return rc;
+    const int B = p->dtype == PRODM_F64 ? 64 : 32;
+    const uint64_t nblk = (p->g.elems[level] + B - 1) / B;
+    const uint64_t head = (p->encoder == PRODM_ENC_GROUPED && plane == 0) ? 4 + nblk : 0;
+    *bytes = head + body;
+    if (!h_dst) return kOk;
+    if (capacity < *bytes) { set_error("plane_stream: destination too small"); return kErrArg; }
+    if (head) {
+        const uint32_t nb32 = (uint32_t)nblk;
+        std::memcpy(h_dst, &nb32, 4);
+        PRODM_HIP(hipMemcpyAsync(h_dst + 4, p->rb + p->rb_off[level], nblk, hipMemcpyDeviceToHost, s));
+    }
+    if (body) PRODM_HIP(hipMemcpyAsync(h_dst + head, src, body, hipMemcpyDeviceToHost, s));
+    PRODM_HIP(hipStreamSynchronize(s));
+    return kOk;
+}
+
+int prodm_mdr_load_plane_stream(prodm_mdr_plan *p, int level, int plane, const uint8_t *h_src, uint64_t bytes,
+                                prodm_stream_t stream) {
+    if (!p || level < 0 || level > p->g.L || plane < 0 || plane >= p->nb || (!h_src && bytes)) {
+        set_error("load_plane_stream: bad arguments");
+        return kErrArg;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const int B = p->dtype == PRODM_F64 ? 64 : 32;
+    const uint64_t nblk = (p->g.elems[level] + B - 1) / B;
+    char *slot = static_cast<char *>(p->planes) + p->elem * (p->plane_off[level] + (uint64_t)plane * p->plane_stride[level]);
+    const uint64_t cap = p->elem * p->plane_stride[level];
+    uint64_t head = 0;
+    if (p->encoder == PRODM_ENC_NEGABINARY) {
+        if (bytes != nblk * p->elem) { set_error("load_plane_stream: negabinary plane size mismatch"); return kErrArg; }
+    } else {
+        if (plane == 0) {
+            uint32_t hdr = 0;
+            if (bytes < 4) { set_error("load_plane_stream: truncated header"); return kErrArg; }
+            std::memcpy(&hdr, h_src, 4);
+            if (hdr != nblk || bytes < 4 + nblk) { set_error("load_plane_stream: header does not match the level"); return kErrArg; }
+            head = 4 + nblk;
+            PRODM_HIP(hipMemcpyAsync(p->rb + p->rb_off[level], h_src + 4, nblk, hipMemcpyHostToDevice, s));
+        }
+        if ((bytes - head) % p->elem) { set_error("load_plane_stream: not a whole number of words"); return kErrArg; }
+    }
+    if (bytes - head > cap) { set_error("load_plane_stream: plane larger than its storage"); return kErrArg; }
+    if (bytes > head) PRODM_HIP(hipMemcpyAsync(slot, h_src + head, bytes - head, hipMemcpyHostToDevice, s));
+    PRODM_HIP(hipStreamSynchronize(s));
+    return kOk;
+}
+
+int prodm_mdr_set_level_max(prodm_mdr_plan *p, int level, double max_abs, prodm_stream_t stream) {
+    if (!p || level < 0 || level > p->g.L || !(max_abs >= 0)) { set_error("set_level_max: bad arguments"); return kErrArg; }
+    hipStream_t s = (hipStream_t)stream;
+    if (p->dtype == PRODM_F32) {
+        const float f = (float)max_abs;
+        uint32_t k;
+        std::memcpy(&k, &f, 4);
+        return launch_fill_u32(reinterpret_cast<uint32_t *>(p->keys) + level, 1, k, s);
+    }
+    uint64_t k;
+    std::memcpy(&k, &max_abs, 8);
+    return launch_fill_u64(reinterpret_cast<uint64_t *>(p->keys) + level, 1, k, s);
+}
+
 int prodm_mdr_retrieve(prodm_mdr_plan *p, const uint8_t *k, void *d_out, prodm_stream_t stream) {
     if (!p || !k || !d_out) { set_error("retrieve: null argument"); return kErrArg; }
     if (p->dtype == PRODM_F32) return plan_retrieve<float>(p, k, static_cast<float *>(d_out), (hipStream_t)stream);

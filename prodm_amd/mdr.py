@@ -261,6 +261,23 @@ class Plan:
         check(lib().prodm_mdr_retrieve(self.h, k, _ptr(out), _stream()), "retrieve")
         return out
 
+    def plane_stream(self, level, p):
+        """Reference-format byte stream of plane p of a level (host bytes)."""
+        n = C.c_uint64()
+        check(lib().prodm_mdr_plane_stream(self.h, level, p, None, 0, C.byref(n), _stream()), "plane stream size")
+        buf = np.zeros(max(int(n.value), 1), np.uint8)
+        check(lib().prodm_mdr_plane_stream(self.h, level, p, buf.ctypes.data_as(C.c_void_p), buf.size, C.byref(n),
+                                           _stream()), "plane stream")
+        return buf[: int(n.value)].tobytes()
+
+    def load_plane_stream(self, level, p, data):
+        arr = np.frombuffer(data, np.uint8) if len(data) else np.zeros(1, np.uint8)
+        check(lib().prodm_mdr_load_plane_stream(self.h, level, p, arr.ctypes.data_as(C.c_void_p), len(data),
+                                                _stream()), "load plane stream")
+
+    def set_level_max(self, level, max_abs):
+        check(lib().prodm_mdr_set_level_max(self.h, level, float(max_abs), _stream()), "set level max")
+
     def profile(self, enable=True):
         check(lib().prodm_mdr_profile(self.h, int(enable)), "profile")
 

@@ -1,0 +1,224 @@
+// test_cpp_pipeline.cpp -- the host C++ layer end to end on the GPU.
+//
+//  1. prodm::ComposedRefactor driving the GPU components and
+//     prodm::GpuRefactor (device-resident) write byte-identical files, for
+//     the negabinary and the grouped encoder.
+//  2. prodm::ComposedReconstructor and prodm::GpuReconstructor read them
+//     progressively; each result meets the requested L-infinity tolerance
+//     and the two agree to rounding.
+//  3. The squared-error tables written with a squared collector drive an
+//     S-norm reconstruction whose actual relative L2 error is reported.
+//  4. GPU components against each other: encode -> progressive decode
+//     equals one-shot decode; interleave -> reposition is the identity.
+//
+// Usage: test_cpp_pipeline <scratch dir>. Exit status 0 on success.
+#include <sys/stat.h>
+
+#include <cmath>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "prodm/prodm.hpp"
+
+using namespace prodm;
+
+static int failures = 0;
+#define EXPECT(cond, ...)                                  \
+    do {                                                   \
+        if (!(cond)) {                                     \
+            std::printf("FAIL %s:%d: ", __FILE__, __LINE__); \
+            std::printf(__VA_ARGS__);                      \
+            std::printf("\n");                             \
+            failures++;                                    \
+        }                                                  \
+    } while (0)
+
+static std::vector<float> field(const std::vector<uint32_t> &d) {
+    std::vector<float> u((size_t)d[0] * d[1] * d[2]);
+    size_t i = 0;
+    for (uint32_t a = 0; a < d[0]; a++)
+        for (uint32_t b = 0; b < d[1]; b++)
+            for (uint32_t c = 0; c < d[2]; c++) {
+                const double x = a / (d[0] - 1.0), y = b / (d[1] - 1.0), z = c / (d[2] - 1.0);
+                double v = std::sin(2 * M_PI * 3 * x) * std::cos(2 * M_PI * 5 * y) * std::sin(2 * M_PI * 2 * z);
+                v += 1.5 * std::exp(-((x - .3) * (x - .3) + (y - .6) * (y - .6) + (z - .4) * (z - .4)) / (2 * .05 * .05));
+                v += 1e-3 * std::sin(12345.678 * (double)i);
+                u[i++] = (float)v;
+            }
+    return u;
+}
+
+static std::vector<std::string> level_files(const std::string &dir, int L) {
+    std::vector<std::string> f;
+    for (int l = 0; l <= L; l++) f.push_back(dir + "/level_" + std::to_string(l) + ".bin");
+    return f;
+}
+
+static std::vector<uint8_t> slurp(const std::string &p) {
+    std::vector<uint8_t> b;
+    FILE *f = std::fopen(p.c_str(), "rb");
+    if (!f) return b;
+    std::fseek(f, 0, SEEK_END);
+    b.resize(std::ftell(f));
+    std::rewind(f);
+    if (!b.empty() && std::fread(b.data(), 1, b.size(), f) != b.size()) b.clear();
+    std::fclose(f);
+    return b;
+}
+
+static double linf(const std::vector<float> &a, const float *b) {
+    double m = 0;
+    for (size_t i = 0; i < a.size(); i++) m = std::max(m, std::fabs((double)a[i] - b[i]));
+    return m;
+}
+
+static double range_of(const std::vector<float> &a) {
+    float lo = a[0], hi = a[0];
+    for (float v : a) lo = std::min(lo, v), hi = std::max(hi, v);
+    return (double)hi - lo;
+}
+
+template <class Encoder>
+static void roundtrip_files(const std::string &root, const std::vector<float> &u, const std::vector<uint32_t> &dims,
+                            int L, int nb, bool nega, Encoder enc) {
+    const std::string a = root + (nega ? "/composed_nb" : "/composed_gb"), b = root + (nega ? "/gpu_nb" : "/gpu_gb");
+    mkdir(a.c_str(), 0755);
+    mkdir(b.c_str(), 0755);
+    Context ctx = make_context();
+    {
+        ComposedRefactor<float, GpuMGARDOrthogonalDecomposer<float>, GpuDirectInterleaver<float>, Encoder,
+                         AdaptiveLevelCompressor, MaxErrorCollector<float>, ConcatLevelFileWriter>
+            r{GpuMGARDOrthogonalDecomposer<float>(ctx), GpuDirectInterleaver<float>(ctx), enc,
+              AdaptiveLevelCompressor(64), MaxErrorCollector<float>(),
+              ConcatLevelFileWriter(a + "/metadata.bin", level_files(a, L))};
+        r.negabinary = nega;
+        r.refactor(u.data(), dims, L, nb);
+    }
+    {
+        GpuRefactor<float, AdaptiveLevelCompressor, ConcatLevelFileWriter> r(
+            nega ? PRODM_ENC_NEGABINARY : PRODM_ENC_GROUPED, AdaptiveLevelCompressor(64),
+            ConcatLevelFileWriter(b + "/metadata.bin", level_files(b, L)));
+        r.refactor(u.data(), dims, L, nb);
+    }
+    EXPECT(slurp(a + "/metadata.bin") == slurp(b + "/metadata.bin") && !slurp(a + "/metadata.bin").empty(),
+           "metadata differs (%s)", nega ? "negabinary" : "grouped");
+    for (int l = 0; l <= L; l++)
+        EXPECT(slurp(level_files(a, L)[l]) == slurp(level_files(b, L)[l]), "level %d file differs (%s)", l,
+               nega ? "negabinary" : "grouped");
+
+    using Est = MaxErrorEstimatorOB<float>;
+    using Interp = GreedyBasedSizeInterpreter<Est>;
+    ComposedReconstructor<float, GpuMGARDOrthogonalDecomposer<float>, GpuDirectInterleaver<float>, Encoder,
+                          AdaptiveLevelCompressor, Interp, Est, ConcatLevelFileRetriever>
+        rc{GpuMGARDOrthogonalDecomposer<float>(ctx), GpuDirectInterleaver<float>(ctx), enc, AdaptiveLevelCompressor(64),
+           Interp(Est(3)), ConcatLevelFileRetriever(a + "/metadata.bin", level_files(a, L))};
+    rc.load_metadata();
+    GpuReconstructor<float, AdaptiveLevelCompressor, Interp, Est, ConcatLevelFileRetriever> rg(
+        AdaptiveLevelCompressor(64), Interp(Est(3)), ConcatLevelFileRetriever(b + "/metadata.bin", level_files(b, L)));
+    rg.load_metadata();
+    const double R = range_of(u);
+    for (double rel : {1e-1, 1e-2, 1e-3, 1e-4, 1e-5}) {
+        const double tol = rel * R;
+        float *x = rc.progressive_reconstruct(tol);
+        std::vector<float> full(x, x + u.size());
+        const bool reduced = rc.get_current_dimensions() != dims;
+        if (reduced) {
+            float *y = rc.recompose_to_full();
+            full.assign(y, y + u.size());
+        }
+        const float *g = rg.progressive_reconstruct(tol);
+        const double ec = linf(u, full.data()), eg = linf(u, g), d = linf(full, g);
+        std::printf("%s tol %.1e: composed err %.3e (%s), gpu err %.3e, retrieved %zu / %zu bytes, |composed-gpu| %.2e\n",
+                    nega ? "negabinary" : "grouped", rel, ec / R, reduced ? "coarse grid" : "full grid", eg / R,
+                    rc.get_retrieved_size(), rg.get_retrieved_size(), d);
+        if (!reduced) EXPECT(ec <= tol, "composed error %.3e above %.3e", ec, tol);
+        EXPECT(eg <= tol, "gpu error %.3e above %.3e", eg, tol);
+        EXPECT(rc.get_retrieved_size() == rg.get_retrieved_size(), "retrieved bytes differ");
+        if (!reduced) EXPECT(d <= 1e-5 * R, "composed and gpu results differ by %.3e", d);
+    }
+}
+
+static void squared_path(const std::string &root, const std::vector<float> &u, const std::vector<uint32_t> &dims, int L,
+                         int nb) {
+    const std::string a = root + "/gpu_sq";
+    mkdir(a.c_str(), 0755);
+    GpuRefactor<float, DefaultLevelCompressor, ConcatLevelFileWriter> r(
+        PRODM_ENC_NEGABINARY, DefaultLevelCompressor(), ConcatLevelFileWriter(a + "/metadata.bin", level_files(a, L)),
+        true);
+    r.refactor(u.data(), dims, L, nb);
+    using Est = SNormErrorEstimator<float>;
+    using Interp = NegaBinaryGreedyBasedSizeInterpreter<Est>;
+    GpuReconstructor<float, DefaultLevelCompressor, Interp, Est, ConcatLevelFileRetriever> rg(
+        DefaultLevelCompressor(), Interp(Est(3, L, 0)), ConcatLevelFileRetriever(a + "/metadata.bin", level_files(a, L)));
+    rg.load_metadata();
+    double norm2 = 0;
+    for (float v : u) norm2 += (double)v * v;
+    for (double rel : {1e-2, 1e-4, 1e-6}) {
+        // the S-norm estimate is in units of squared coefficient error
+        // times level volume; ask for (rel * ||u||)^2 over the grid
+        const double tol = rel * rel * norm2;
+        const float *g = rg.progressive_reconstruct(tol);
+        EXPECT(g != nullptr, "no squared tables");
+        if (!g) return;
+        double e2 = 0;
+        for (size_t i = 0; i < u.size(); i++) e2 += ((double)u[i] - g[i]) * ((double)u[i] - g[i]);
+        std::printf("L2 target %.0e: actual relative L2 %.3e, retrieved %zu bytes\n", rel, std::sqrt(e2 / norm2),
+                    rg.get_retrieved_size());
+    }
+}
+
+template <class Encoder>
+static void component_checks(const std::vector<float> &u, int nb, Encoder enc) {
+    const int n = (int)u.size() / 7 + 5;
+    float mx = 0;
+    for (int i = 0; i < n; i++) mx = std::max(mx, std::fabs(u[i]));
+    int e;
+    std::frexp(mx, &e);
+    std::vector<uint32_t> sizes;
+    auto streams = enc.encode(u.data(), n, e, nb, sizes);
+    std::vector<const uint8_t *> cs(streams.begin(), streams.end());
+    float *one = enc.decode(cs, n, e, nb);
+    std::vector<double> acc(n, 0.0);
+    const int cuts[] = {0, 3, nb / 2, nb};
+    for (int c = 0; c < 3; c++) {
+        std::vector<const uint8_t *> part(cs.begin() + cuts[c], cs.begin() + cuts[c + 1]);
+        float *d = enc.progressive_decode(part, n, e, cuts[c], cuts[c + 1] - cuts[c], 5);
+        for (int i = 0; i < n; i++) acc[i] += d[i];
+        std::free(d);
+    }
+    double worst = 0;
+    for (int i = 0; i < n; i++) worst = std::max(worst, std::fabs(acc[i] - one[i]));
+    EXPECT(worst <= std::ldexp(1.0, e - nb + 3), "progressive deltas differ from one-shot decode by %.3e", worst);
+    std::free(one);
+    for (auto *s : streams) std::free(s);
+}
+
+int main(int argc, char **argv) {
+    const std::string root = argc > 1 ? argv[1] : "cpp_pipeline_out";
+    mkdir(root.c_str(), 0755);
+    const std::vector<uint32_t> dims = {65, 65, 65};
+    const int L = 4, nb = 32;
+    const auto u = field(dims);
+    Context ctx = make_context();
+    roundtrip_files(root, u, dims, L, nb, true, GpuNegaBinaryBPEncoder<float, uint32_t>(ctx));
+    roundtrip_files(root, u, dims, L, nb, false, GpuGroupedBPEncoder<float, uint32_t>(ctx));
+    squared_path(root, u, dims, L, nb);
+    component_checks(u, nb, GpuNegaBinaryBPEncoder<float, uint32_t>(ctx));
+    component_checks(u, nb, GpuGroupedBPEncoder<float, uint32_t>(ctx));
+    {  // interleave -> reposition is the identity on every level
+        GpuDirectInterleaver<float> ilv(ctx);
+        const auto ld = level_dims_of(dims, L);
+        std::vector<float> back(u.size(), 0.f);
+        const std::vector<uint32_t> zero(3, 0);
+        const auto el = level_elements_of(dims, L);
+        for (int l = 0; l <= L; l++) {
+            std::vector<float> buf(el[l]);
+            ilv.interleave(u.data(), dims, ld[l], l ? ld[l - 1] : zero, buf.data());
+            ilv.reposition(buf.data(), dims, ld[l], l ? ld[l - 1] : zero, back.data());
+        }
+        EXPECT(back == u, "interleave/reposition is not the identity");
+    }
+    std::printf(failures ? "FAILED (%d)\n" : "PASSED\n", failures);
+    return failures ? 1 : 0;
+}
