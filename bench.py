@@ -12,7 +12,6 @@ value = 2 * input bytes / (t_refactor + t_retrieve)  [GB/s]
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
-import heapq
 import json
 import math
 import os
@@ -54,40 +53,39 @@ def make_field_gpu(n, seed=0, nbumps=16, noise=1e-4):
     return u.to(torch.float32).contiguous().view(-1)
 
 
-def greedy_planes(level_bounds, level_plane_bytes, nb, tol, c):
-    """GreedyBasedSizeInterpreter::interpret_retrieve_size
-    (GreedyBasedSizeInterpreter.hpp:168-204) over MaxErrorCollector tables
-    (MaxErrorCollector.hpp:14-25) with MaxErrorEstimatorOB (float estimator,
-    MaxErrorEstimator.hpp:13-40). Returns planes per level."""
-    f32 = np.float32
-    L1 = len(level_bounds)
+def plan_planes(level_bounds, level_plane_bytes, nb, tol):
+    """Planes per level for an L-infinity tolerance: the host C++ greedy
+    planner (GreedyBasedSizeInterpreter.hpp:23-70) over the MaxErrorCollector
+    tables (MaxErrorCollector.hpp:14-25) with MaxErrorEstimatorOB(3)
+    (MaxErrorEstimator.hpp:13-40), as ComposedReconstructor plans a first
+    retrieval (ComposedReconstructor.hpp:33-53)."""
+    import ctypes as C
+    from prodm_amd import capi, interpret
     tables = []
     for b in level_bounds:
-        _, e = math.frexp(b)
-        t = [float(b)] + [math.ldexp(1.0, e - k) for k in range(1, nb + 1)]
+        t = np.zeros(nb + 1)
+        capi.lib().prodm_collect_max_f32(C.c_float(b), nb, t.ctypes.data_as(C.c_void_p))
         tables.append(t)
-    est = lambda err: float(f32(c) * f32(err))
-    idx = [0] * L1
-    acc = 0.0
-    for i in range(L1):
-        acc += est(tables[i][0])
-    heap = []
-    for i in range(L1):
-        gain = float(f32(c) * (f32(tables[i][0]) - f32(tables[i][1])))
-        heapq.heappush(heap, (-(gain / level_plane_bytes[i]), i))
-    met = False
-    while not met and heap:
-        _, i = heapq.heappop(heap)
-        j = idx[i]
-        acc -= est(tables[i][j])
-        acc += est(tables[i][j + 1])
-        if acc < tol:
-            met = True
-        idx[i] += 1
-        if idx[i] != nb:
-            gain = float(f32(c) * (f32(tables[i][idx[i]]) - f32(tables[i][idx[i] + 1])))
-            heapq.heappush(heap, (-(gain / level_plane_bytes[i]), i))
-    return idx, acc
+    sizes = [[pb] * nb for pb in level_plane_bytes]
+    idx = np.zeros(len(level_bounds), np.uint8)
+    _, idx = interpret.retrieve_sizes(interpret.GREEDY, interpret.MAX_OB, sizes, tables, tol, idx, ndims=3)
+    c = interpret.max_estimator_ob_c(3)
+    est = sum(float(np.float32(c) * np.float32(tables[l][idx[l]])) for l in range(len(level_bounds)))
+    return [int(k) for k in idx], est
+
+
+def calibrate_copy(mib=512, reps=3):
+    """A plain copy kernel of known traffic (torch elementwise y = x + 0 over
+    `mib` MiB of fp32: reads and writes `mib` MiB per launch), for scaling
+    the FETCH_SIZE / WRITE_SIZE counters of the MDR kernels."""
+    import torch
+    x = torch.ones(mib * 2 ** 18, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        torch.add(x, 0.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read_per_launch": mib * 2 ** 20,
+                      "bytes_written_per_launch": mib * 2 ** 20, "launches": reps}))
 
 
 def cpu_baseline(u_host, dims, L, nb, planes):
@@ -127,7 +125,12 @@ def main():
     ap.add_argument("--bitplanes", type=int, default=32)
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run only a plain copy kernel of known traffic (for counter scaling) and exit")
     args = ap.parse_args()
+    if args.calibrate:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU layout is described in "
               "DESIGN.md and is not run.", file=sys.stderr)
@@ -157,8 +160,7 @@ def main():
         m, e, el = plan.level_info(l)
         bounds.append(float(np.float32(m) * np.float32(4.0)))
         plane_bytes.append(((el + 31) // 32) * 4)
-    c_ob = float(np.float32((1.0 + 21.0 * math.sqrt(3) / 8) * 4))
-    planes, est = greedy_planes(bounds, plane_bytes, nb, tau, c_ob)
+    planes, est = plan_planes(bounds, plane_bytes, nb, tau)
     log(f"[bench] tau={tau:.4e} planes per level {planes} (estimated error {est:.4e})")
     plan.retrieve(planes, out)
     torch.cuda.synchronize()
