@@ -246,7 +246,7 @@ class GpuGroupedBPEncoder {
                           uint8_t k, int level) {
         DeviceContext &c = *ctx_;
         State &s = *st_;
-        if (!s.dev) check(prodm_grouped_state_create(&s.dev, 256, dtype_of<T>()), "grouped state");
+        if (!s.dev) check(prodm_grouped_state_create(&s.dev, 64, dtype_of<T>()), "grouped state");  // levels 0..63
         const uint64_t nblk = blocks(n);
         auto &hist = s.rb_hist[level];
         if (k && start == 0) {  // stream 0 carries u32 nblocks + rb[nblocks]

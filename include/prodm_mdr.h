@@ -60,6 +60,18 @@ int prodm_recompose_f32(const float *d_in, float *d_out, const uint32_t *dims, i
 int prodm_recompose_f64(const double *d_in, double *d_out, const uint32_t *dims, int ndims, int target_level,
                         const uint32_t *strides, void *d_ws, size_t ws_bytes, prodm_stream_t stream);
 
+/* ---- Adjoint (transpose) of recompose, FP64 inside. No counterpart in
+ *      ProDM (whose reconstructor only applies recompose,
+ *      ComposedReconstructor.hpp:214,250-253); it gives the sensitivity of
+ *      a quantity of interest J to every stored coefficient: d_g = dJ/du on
+ *      the box (dims, strides), d_out = dJ/dcoefficients in the layout of
+ *      prodm_decompose's output (same dims/strides).                      */
+size_t prodm_recompose_adjoint_workspace_bytes(const uint32_t *dims, int ndims, int target_level);
+int prodm_recompose_adjoint_f32(const float *d_g, float *d_out, const uint32_t *dims, int ndims, int target_level,
+                                const uint32_t *strides, void *d_ws, size_t ws_bytes, prodm_stream_t stream);
+int prodm_recompose_adjoint_f64(const double *d_g, double *d_out, const uint32_t *dims, int ndims, int target_level,
+                                const uint32_t *strides, void *d_ws, size_t ws_bytes, prodm_stream_t stream);
+
 /* ---- Interleaver: DirectInterleaver<T>::interleave / reposition
  *      (include/MDR/Interleaver/DirectInterleaver.hpp:12-63); ndims 1..3. */
 int prodm_interleave_f32(const float *d_data, const uint32_t *dims, int ndims, const uint32_t *dims_fine,
@@ -112,7 +124,7 @@ int prodm_grouped_encode_f64(const double *d_x, uint32_t n, int32_t exp, int num
                              uint64_t *d_planes, uint64_t plane_stride_words, uint32_t *d_words, void *d_ws,
                              size_t ws_bytes, prodm_stream_t stream);
 typedef struct prodm_grouped_state prodm_grouped_state;
-int prodm_grouped_state_create(prodm_grouped_state **out, int max_levels, int dtype);
+int prodm_grouped_state_create(prodm_grouped_state **out, int max_levels, int dtype);  /* max_levels 1..64 */
 int prodm_grouped_state_destroy(prodm_grouped_state *state);
 int prodm_grouped_decode_f32(prodm_grouped_state *state, int level, const uint8_t *const *h_stream_ptrs, uint32_t n,
                              int32_t exp, int start, int k, float *d_out, prodm_stream_t stream);

@@ -27,6 +27,9 @@ _SIGS = {
     "prodm_decompose_f64": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp, vp, sz, vp]),
     "prodm_recompose_f32": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp, vp, sz, vp]),
     "prodm_recompose_f64": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp, vp, sz, vp]),
+    "prodm_recompose_adjoint_workspace_bytes": (sz, [vp, C.c_int, C.c_int]),
+    "prodm_recompose_adjoint_f32": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp, vp, sz, vp]),
+    "prodm_recompose_adjoint_f64": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp, vp, sz, vp]),
     "prodm_interleave_f32": (C.c_int, [vp, vp, C.c_int, vp, vp, vp, vp, vp]),
     "prodm_interleave_f64": (C.c_int, [vp, vp, C.c_int, vp, vp, vp, vp, vp]),
     "prodm_reposition_f32": (C.c_int, [vp, vp, C.c_int, vp, vp, vp, vp, vp]),

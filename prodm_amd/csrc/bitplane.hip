@@ -1,19 +1,20 @@
 // bitplane.hip -- bitplane encoders/decoders and max-abs for gfx950.
 //
-// Layout: a wavefront owns a chunk of 64 "groups" of 64 consecutive
-// elements. For each group and each plane one __ballot yields the 64-bit
-// plane word of those 64 elements (two 32-element blocks for u32 streams,
-// one 64-element block for u64 streams); lane c keeps the words of group c,
-// so the final stores are coalesced 64-bit per lane. Bit i of a word is
-// element i of its block, LSB first (GroupedBPEncoder.hpp:345,
-// NegaBinaryBPEncoder.hpp:246).
+// Layout: one lane per bitplane block (32 elements for u32 words, 64 for
+// u64). A workgroup stages its blocks through LDS with coalesced global
+// accesses (rows padded by one word so that each lane's block is read or
+// written without bank conflicts), and each lane turns its block into plane
+// words with an in-register bit-matrix transpose: after transpose32, word k
+// holds bit k of every element, LSB = first element of the block
+// (GroupedBPEncoder.hpp:345, NegaBinaryBPEncoder.hpp:246). Plane stores and
+// loads are then one coalesced word per lane per plane.
+#include <type_traits>
+
 #include "internal.hpp"
 
 namespace prodm {
 
 constexpr int kThreads = 256;
-constexpr int kWavesPerBlock = kThreads / kWave;
-constexpr uint32_t kChunk = 64 * 64;  // elements per wave
 
 // ------------------------------------------------------------------------------
 // fills
@@ -72,6 +73,110 @@ template <class T> __device__ inline int level_exp(int32_t exp, const KeyOf<T> *
 }
 
 // ------------------------------------------------------------------------------
+// bit-matrix transposes: A[r] bit c  ->  A[c] bit r
+// ------------------------------------------------------------------------------
+__device__ __forceinline__ void transpose32(uint32_t (&A)[32]) {
+#pragma unroll
+    for (int s = 0; s < 5; s++) {
+        const int j = 16 >> s;
+        const uint32_t m = s == 0 ? 0x0000FFFFu : s == 1 ? 0x00FF00FFu : s == 2 ? 0x0F0F0F0Fu : s == 3 ? 0x33333333u : 0x55555555u;
+#pragma unroll
+        for (int k = 0; k < 32; k++) {
+            if ((k & j) == 0) {
+                const uint32_t t = ((A[k] >> j) ^ A[k + j]) & m;
+                A[k] ^= t << j;
+                A[k + j] ^= t;
+            }
+        }
+    }
+}
+
+template <int OFF, int N>
+__device__ __forceinline__ void transpose32_at(uint32_t (&A)[N]) {
+#pragma unroll
+    for (int s = 0; s < 5; s++) {
+        const int j = 16 >> s;
+        const uint32_t m = s == 0 ? 0x0000FFFFu : s == 1 ? 0x00FF00FFu : s == 2 ? 0x0F0F0F0Fu : s == 3 ? 0x33333333u : 0x55555555u;
+#pragma unroll
+        for (int k = 0; k < 32; k++) {
+            if ((k & j) == 0) {
+                const uint32_t t = ((A[OFF + k] >> j) ^ A[OFF + k + j]) & m;
+                A[OFF + k] ^= t << j;
+                A[OFF + k + j] ^= t;
+            }
+        }
+    }
+}
+
+// 64 x 64 as four in-place 32 x 32 blocks. Element-major layout: element e
+// is (hi[e] << 32) | lo[e]. Word-major layout: word w (bit w of the 64
+// elements, element 0 in bit 0) is (w < 32) ? lo[w] | lo[32 + w] << 32
+// : hi[w - 32] | hi[w] << 32. The transform is its own inverse.
+__device__ __forceinline__ void transpose64(uint32_t (&lo)[64], uint32_t (&hi)[64]) {
+    transpose32_at<0>(lo);
+    transpose32_at<32>(lo);
+    transpose32_at<0>(hi);
+    transpose32_at<32>(hi);
+}
+template <int W> __device__ __forceinline__ uint64_t word64_get(const uint32_t (&lo)[64], const uint32_t (&hi)[64]) {
+    if constexpr (W < 32) return ((uint64_t)lo[32 + W] << 32) | lo[W];
+    else return ((uint64_t)hi[W] << 32) | hi[W - 32];
+}
+template <int W> __device__ __forceinline__ void word64_set(uint32_t (&lo)[64], uint32_t (&hi)[64], uint64_t v) {
+    if constexpr (W < 32) { lo[W] = (uint32_t)v; lo[32 + W] = (uint32_t)(v >> 32); }
+    else { hi[W - 32] = (uint32_t)v; hi[W] = (uint32_t)(v >> 32); }
+}
+template <int I, int N, class F> __device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+// ------------------------------------------------------------------------------
+// block staging geometry
+// ------------------------------------------------------------------------------
+template <class T> struct BlkStage;
+template <> struct BlkStage<float> {
+    static constexpr int kTPB = 256;                // threads (= blocks) per workgroup
+    static constexpr int kB = 32;                   // elements per block
+    static constexpr int kPitch = kB + 1;           // LDS row pitch (words)
+};
+template <> struct BlkStage<double> {
+    static constexpr int kTPB = 64;
+    static constexpr int kB = 64;
+    static constexpr int kPitch = kB + 1;
+};
+
+// coalesced load of the workgroup's TPB*B elements into LDS rows (0 past n)
+template <class T>
+__device__ inline void stage_in(const T *__restrict__ x, uint64_t n, uint64_t e0, T *sx) {
+    constexpr int TPB = BlkStage<T>::kTPB, B = BlkStage<T>::kB, PI = BlkStage<T>::kPitch;
+    // all B loads in flight before the first LDS store
+    T v[B];
+#pragma unroll
+    for (int i = 0; i < B; i++) {
+        const uint64_t e = e0 + (uint32_t)(i * TPB) + threadIdx.x;
+        v[i] = e < n ? x[e] : (T)0;
+    }
+#pragma unroll
+    for (int i = 0; i < B; i++) {
+        const uint32_t le = i * TPB + threadIdx.x;
+        sx[(le / B) * PI + (le % B)] = v[i];
+    }
+}
+template <class T>
+__device__ inline void stage_out(T *__restrict__ out, uint64_t n, uint64_t e0, const T *sx) {
+    constexpr int TPB = BlkStage<T>::kTPB, B = BlkStage<T>::kB, PI = BlkStage<T>::kPitch;
+#pragma unroll 8
+    for (int i = 0; i < B; i++) {
+        const uint32_t le = i * TPB + threadIdx.x;
+        const uint64_t e = e0 + le;
+        if (e < n) out[e] = sx[(le / B) * PI + (le % B)];
+    }
+}
+
+// ------------------------------------------------------------------------------
 // NegaBinary
 // ------------------------------------------------------------------------------
 // binary2negabinary of (T_fps) v (NegaBinaryBPEncoder.hpp:43,219-224)
@@ -83,53 +188,47 @@ template <class T> __device__ inline typename Traits<T>::Fixed to_negabinary(T v
     return ((F)q + (F)Traits<T>::kNbMask) ^ (F)Traits<T>::kNbMask;
 }
 
-template <class T, int MAXNB>
-__global__ __launch_bounds__(kThreads) void k_nb_encode(const T *__restrict__ x, uint32_t n, int32_t exp,
-                                                        const KeyOf<T> *key, int nb,
-                                                        typename Traits<T>::Stream *__restrict__ planes,
-                                                        uint64_t stride) {
-    using F = typename Traits<T>::Fixed;
-    const uint32_t lane = lane_id();
-    const uint64_t wave = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-    const uint64_t base = wave * kChunk;
-    if (base >= n) return;
+template <class T>
+__global__ __launch_bounds__(BlkStage<T>::kTPB) void k_nb_encode(const T *__restrict__ x, uint32_t n, int32_t exp,
+                                                              const KeyOf<T> *key, int nb,
+                                                              typename Traits<T>::Stream *__restrict__ planes,
+                                                              uint64_t stride) {
+    constexpr int TPB = BlkStage<T>::kTPB, B = BlkStage<T>::kB, PI = BlkStage<T>::kPitch;
+    __shared__ T sx[TPB * PI];
+    const uint64_t b0 = (uint64_t)blockIdx.x * TPB;
+    stage_in<T>(x, n, b0 * B, sx);
+    __syncthreads();
     const int shift = nb - (level_exp<T>(exp, key) + 2);  // exp += 2 (:21)
-    uint64_t w[MAXNB];
+    const uint64_t b = b0 + threadIdx.x;
+    const uint64_t nw = ((uint64_t)n + B - 1) / B;
+    const T *row = sx + threadIdx.x * PI;
+    if constexpr (sizeof(T) == 4) {
+        uint32_t q[32];
 #pragma unroll
-    for (int p = 0; p < MAXNB; p++) w[p] = 0;
-    for (int c0 = 0; c0 < 64; c0 += 8) {
-        T v[8];
+        for (int j = 0; j < 32; j++) q[j] = to_negabinary<T>(dev_ldexp(row[j], shift));
+        transpose32(q);
+        if (b < nw) {
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            uint64_t e = base + (uint64_t)(c0 + u) * 64 + lane;
-            v[u] = e < n ? x[e] : (T)0;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            F q = to_negabinary<T>(dev_ldexp(v[u], shift));
-#pragma unroll
-            for (int p = 0; p < MAXNB; p++) {
-                if (p < nb) {
-                    uint64_t m = ballot64((q >> (nb - 1 - p)) & 1u);
-                    w[p] = (lane == (uint32_t)(c0 + u)) ? m : w[p];
-                }
+            for (int k = 0; k < 32; k++) {
+                const int p = nb - 1 - k;  // plane p holds bit nb - 1 - p
+                if (p >= 0) planes[(uint64_t)p * stride + b] = q[k];
             }
         }
-    }
-    const uint64_t g = base / 64 + lane;  // group index
-    if constexpr (sizeof(T) == 4) {
-        const uint64_t nw = ((uint64_t)n + 31) / 32;
-        if (2 * g < nw) {
-#pragma unroll
-            for (int p = 0; p < MAXNB; p++)
-                if (p < nb) reinterpret_cast<uint64_t *>(planes + (uint64_t)p * stride)[g] = w[p];
-        }
     } else {
-        const uint64_t nw = ((uint64_t)n + 63) / 64;
-        if (g < nw) {
+        uint32_t lo[64], hi[64];
 #pragma unroll
-            for (int p = 0; p < MAXNB; p++)
-                if (p < nb) planes[(uint64_t)p * stride + g] = w[p];
+        for (int j = 0; j < 64; j++) {
+            const uint64_t q = to_negabinary<T>(dev_ldexp(row[j], shift));
+            lo[j] = (uint32_t)q;
+            hi[j] = (uint32_t)(q >> 32);
+        }
+        transpose64(lo, hi);
+        if (b < nw) {
+            static_for<0, 64>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                const int p = nb - 1 - k;
+                if (p >= 0) planes[(uint64_t)p * stride + b] = word64_get<k>(lo, hi);
+            });
         }
     }
 }
@@ -140,17 +239,9 @@ int launch_nb_encode(const T *x, uint32_t n, int32_t exp, const KeyOf<T> *key, i
     constexpr int B = Traits<T>::kBlock;
     if (n == 0 || nb < 1 || nb > B) { set_error("negabinary encode: bad n or num_bitplanes"); return kErrArg; }
     const uint64_t nw = ((uint64_t)n + B - 1) / B;
-    if (sizeof(T) == 4 && ((stride & 1) || stride < ((nw + 1) & ~1ull))) {
-        set_error("negabinary encode: plane stride must be even and >= words rounded up to even");
-        return kErrArg;
-    }
-    if (sizeof(T) == 8 && stride < nw) { set_error("negabinary encode: plane stride too small"); return kErrArg; }
-    const uint32_t waves = div_up(n, kChunk);
-    const uint32_t grid = div_up(waves, kWavesPerBlock);
-    if (nb <= 32)
-        hipLaunchKernelGGL((k_nb_encode<T, 32>), dim3(grid), dim3(kThreads), 0, s, x, n, exp, key, nb, planes, stride);
-    else
-        hipLaunchKernelGGL((k_nb_encode<T, 64>), dim3(grid), dim3(kThreads), 0, s, x, n, exp, key, nb, planes, stride);
+    if (stride < nw) { set_error("negabinary encode: plane stride below the words per plane"); return kErrArg; }
+    const uint32_t grid = div_up(nw, BlkStage<T>::kTPB);
+    hipLaunchKernelGGL(k_nb_encode<T>, dim3(grid), dim3(BlkStage<T>::kTPB), 0, s, x, n, exp, key, nb, planes, stride);
     PRODM_CHECK_LAUNCH();
     return kOk;
 }
@@ -159,57 +250,54 @@ template int launch_nb_encode<float>(const float *, uint32_t, int32_t, const uin
 template int launch_nb_encode<double>(const double *, uint32_t, int32_t, const unsigned long long *, int, uint64_t *,
                                       uint64_t, hipStream_t);
 
-// plane word (64 bits covering the 64 elements of group g) of plane i
 template <class T>
-__device__ inline uint64_t group_word_lane_load(const PlanePtrs &pl, int k, uint64_t g, uint64_t nw, uint32_t lane) {
-    // u32 streams: lane 2i / 2i+1 load the low / high word of plane i
-    // u64 streams: lane i loads the word of plane i
-    if constexpr (sizeof(T) == 4) {
-        const uint32_t i = lane >> 1;
-        const uint64_t wi = 2 * g + (lane & 1u);
-        if ((int)i < k && wi < nw) return static_cast<const uint32_t *>(pl.p[i])[wi];
-        return 0;
-    } else {
-        if ((int)lane < k && g < nw) return static_cast<const uint64_t *>(pl.p[lane])[g];
-        return 0;
-    }
-}
-
-template <class T>
-__device__ inline uint64_t group_word(uint64_t v, int i) {
-    if constexpr (sizeof(T) == 4) {
-        uint32_t lo = readlane_u32((uint32_t)v, 2 * i), hi = readlane_u32((uint32_t)v, 2 * i + 1);
-        return ((uint64_t)hi << 32) | lo;
-    } else {
-        return readlane_u64(v, i);
-    }
-}
-
-template <class T>
-__global__ __launch_bounds__(kThreads) void k_nb_decode(PlanePtrs pl, uint32_t n, int32_t exp, const KeyOf<T> *key,
-                                                        int start, int k, T *__restrict__ out) {
-    using F = typename Traits<T>::Fixed;
+__global__ __launch_bounds__(BlkStage<T>::kTPB) void k_nb_decode(PlanePtrs pl, uint32_t n, int32_t exp,
+                                                              const KeyOf<T> *key, int start, int k,
+                                                              T *__restrict__ out) {
+    using S = typename Traits<T>::Stream;
     using SF = typename Traits<T>::SFixed;
-    constexpr int B = Traits<T>::kBlock;
-    const uint32_t lane = lane_id();
-    const uint64_t ngroups = ((uint64_t)n + 63) / 64;
+    constexpr int TPB = BlkStage<T>::kTPB, B = BlkStage<T>::kB, PI = BlkStage<T>::kPitch;
+    __shared__ T sx[TPB * PI];
+    const uint64_t b0 = (uint64_t)blockIdx.x * TPB;
+    const uint64_t b = b0 + threadIdx.x;
     const uint64_t nw = ((uint64_t)n + B - 1) / B;
     const int e2 = level_exp<T>(exp, key) + 2;
     const int end = start + k;
-    const uint64_t wave0 = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-    const uint64_t nwaves = (uint64_t)gridDim.x * kWavesPerBlock;
-    for (uint64_t g = wave0; g < ngroups; g += nwaves) {
-        const uint64_t v = group_word_lane_load<T>(pl, k, g, nw, lane);
-        F acc = 0;
-        for (int i = 0; i < k; i++) {
-            const uint64_t word = group_word<T>(v, i);
-            acc |= (F)((word >> lane) & 1u) << (k - 1 - i);
+    T *row = sx + threadIdx.x * PI;
+    // acc bit (k - 1 - i) comes from plane start + i
+    if constexpr (sizeof(T) == 4) {
+        uint32_t a[32];
+#pragma unroll
+        for (int r = 0; r < 32; r++) {
+            const int i = k - 1 - r;
+            a[r] = (i >= 0 && b < nw) ? static_cast<const S *>(pl.p[i])[b] : 0u;
         }
-        const SF val = (SF)((acc ^ (F)Traits<T>::kNbMask) - (F)Traits<T>::kNbMask);
-        const T y = dev_ldexp((T)val, e2 - end);
-        const uint64_t e = g * 64 + lane;
-        if (e < n) out[e] = (end & 1) ? -y : y;
+        transpose32(a);
+#pragma unroll
+        for (int j = 0; j < 32; j++) {
+            const SF val = (SF)((a[j] ^ (uint32_t)Traits<T>::kNbMask) - (uint32_t)Traits<T>::kNbMask);
+            const T y = dev_ldexp((T)val, e2 - end);
+            row[j] = (end & 1) ? -y : y;  // odd ending plane flips the sign (:153,172-189)
+        }
+    } else {
+        uint32_t lo[64], hi[64];
+        static_for<0, 64>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            const int i = k - 1 - r;
+            const uint64_t w = (i >= 0 && b < nw) ? static_cast<const S *>(pl.p[i])[b] : 0ull;
+            word64_set<r>(lo, hi, w);
+        });
+        transpose64(lo, hi);
+#pragma unroll
+        for (int j = 0; j < 64; j++) {
+            const uint64_t acc = ((uint64_t)hi[j] << 32) | lo[j];
+            const SF val = (SF)((acc ^ (uint64_t)Traits<T>::kNbMask) - (uint64_t)Traits<T>::kNbMask);
+            const T y = dev_ldexp((T)val, e2 - end);
+            row[j] = (end & 1) ? -y : y;
+        }
     }
+    __syncthreads();
+    stage_out<T>(out, n, b0 * B, sx);
 }
 
 template <class T>
@@ -218,14 +306,14 @@ int launch_nb_decode(const PlanePtrs &planes, uint32_t n, int32_t exp, const Key
     constexpr int B = Traits<T>::kBlock;
     if (n == 0) return kOk;
     if (k < 0 || start < 0 || start + k > B) { set_error("negabinary decode: bad plane range"); return kErrArg; }
-    const uint64_t ngroups = ((uint64_t)n + 63) / 64;
-    const uint32_t grid = (uint32_t)std::min<uint64_t>(div_up(ngroups, kWavesPerBlock), 16384);
     if (k == 0) {
         // k == 0: the reference returns zeros (:135-138)
         if constexpr (sizeof(T) == 4) return launch_fill_u32(reinterpret_cast<uint32_t *>(out), n, 0u, s);
         else return launch_fill_u64(reinterpret_cast<uint64_t *>(out), n, 0ull, s);
     }
-    hipLaunchKernelGGL(k_nb_decode<T>, dim3(grid), dim3(kThreads), 0, s, planes, n, exp, key, start, k, out);
+    const uint64_t nw = ((uint64_t)n + B - 1) / B;
+    const uint32_t grid = div_up(nw, BlkStage<T>::kTPB);
+    hipLaunchKernelGGL(k_nb_decode<T>, dim3(grid), dim3(BlkStage<T>::kTPB), 0, s, planes, n, exp, key, start, k, out);
     PRODM_CHECK_LAUNCH();
     return kOk;
 }
@@ -265,47 +353,37 @@ template <class F> __device__ inline int rb_of(F orv, int nb) {
     return nb - 1 - msb;
 }
 
-// blocks per wave chunk: 128 (u32) or 64 (u64)
-template <class T> constexpr uint32_t blocks_per_wave() { return kChunk / Traits<T>::kBlock; }
-template <class T> constexpr uint32_t blocks_per_wg() { return blocks_per_wave<T>() * kWavesPerBlock; }
-
 // Pass 1: rb per block and an rb histogram per workgroup (hist[wg][0..nb]).
 template <class T>
-__global__ __launch_bounds__(kThreads) void k_gb_rb(const T *__restrict__ x, uint32_t n, int32_t exp,
-                                                    const KeyOf<T> *key, int nb, int compat, uint8_t *__restrict__ rb,
-                                                    uint32_t *__restrict__ hist) {
+__global__ __launch_bounds__(BlkStage<T>::kTPB) void k_gb_rb(const T *__restrict__ x, uint32_t n, int32_t exp,
+                                                          const KeyOf<T> *key, int nb, int compat,
+                                                          uint8_t *__restrict__ rb, uint32_t *__restrict__ hist) {
     using F = typename Traits<T>::Fixed;
-    constexpr int B = Traits<T>::kBlock;
+    constexpr int TPB = BlkStage<T>::kTPB, B = BlkStage<T>::kB, PI = BlkStage<T>::kPitch;
+    __shared__ T sx[TPB * PI];
     __shared__ uint32_t h[65];
-    for (int i = threadIdx.x; i <= nb; i += kThreads) h[i] = 0;
+    for (int i = threadIdx.x; i <= nb; i += TPB) h[i] = 0;
+    const uint64_t b0 = (uint64_t)blockIdx.x * TPB;
+    stage_in<T>(x, n, b0 * B, sx);
     __syncthreads();
-    const uint32_t lane = lane_id();
-    const uint64_t wave = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-    const uint64_t base = wave * kChunk;
-    const uint32_t nblk = (n - 1) / B + 1;
     const int shift = nb - level_exp<T>(exp, key);
-    if (base < n) {
-        for (int c = 0; c < 64; c++) {
-            const uint64_t e = base + (uint64_t)c * 64 + lane;
-            F m = e < n ? gb_mag<T>(x[e], shift, compat) : (F)0;
-            // OR over the block (32 or 64 lanes)
-            for (int off = B / 2; off >= 1; off >>= 1) m |= __shfl_xor(m, off, 64);
-            if ((lane % B) == 0) {
-                const uint64_t b = (base + (uint64_t)c * 64 + lane) / B;
-                if (b < nblk) {
-                    const int r = rb_of<F>(m, nb);
-                    rb[b] = (uint8_t)r;
-                    atomicAdd(&h[r], 1u);
-                }
-            }
-        }
+    const uint64_t b = b0 + threadIdx.x;
+    const uint32_t nblk = (n - 1) / B + 1;
+    const T *row = sx + threadIdx.x * PI;
+    F orv = 0;
+#pragma unroll 8
+    for (int j = 0; j < B; j++) orv |= gb_mag<T>(row[j], shift, compat);
+    if (b < nblk) {
+        const int r = rb_of<F>(orv, nb);
+        rb[b] = (uint8_t)r;
+        atomicAdd(&h[r], 1u);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i <= nb; i += kThreads) hist[(uint64_t)blockIdx.x * (nb + 1) + i] = h[i];
+    for (int i = threadIdx.x; i <= nb; i += TPB) hist[(uint64_t)blockIdx.x * (nb + 1) + i] = h[i];
 }
 
-// words of block-group counts per plane p: #(rb <= p) + #(rb == p)
-// One workgroup per plane scans cnt over workgroups: off[wg][i] exclusive.
+// words per plane p of each workgroup: #(rb <= p) + #(rb == p). One
+// workgroup per plane scans these over workgroups: off[wg][i] exclusive.
 __global__ __launch_bounds__(1024) void k_gb_scan(const uint32_t *__restrict__ hist, uint32_t nwg, int nbins,
                                                   int pstart, int k, uint32_t *__restrict__ off,
                                                   uint32_t *__restrict__ totals) {
@@ -337,88 +415,97 @@ __global__ __launch_bounds__(1024) void k_gb_scan(const uint32_t *__restrict__ h
     if (threadIdx.x == 0) totals[i] = carry;
 }
 
+// in-workgroup exclusive position of this lane's words for plane p, given
+// its counts (c1 = rb <= p, c2 = rb == p): wave prefix by ballot + mbcnt,
+// cross-wave prefix through LDS. All threads must call it (barriers).
+template <int TPB>
+__device__ inline uint32_t wg_prefix(bool c1, bool c2, uint32_t *wtot) {
+    const uint32_t wv = threadIdx.x / 64;
+    const uint64_t m1 = ballot64(c1), m2 = ballot64(c2);
+    if (lane_id() == 0) wtot[wv] = __popcll(m1) + __popcll(m2);
+    __syncthreads();
+    uint32_t pre = lanes_below(m1) + lanes_below(m2);
+    for (uint32_t q = 0; q < wv; q++) pre += wtot[q];
+    __syncthreads();
+    return pre;
+}
+
 // Pass 2: plane words, compacted per plane.
-template <class T, int MAXNB>
-__global__ __launch_bounds__(kThreads) void k_gb_write(const T *__restrict__ x, uint32_t n, int32_t exp,
-                                                       const KeyOf<T> *key, int nb, int compat,
-                                                       const uint32_t *__restrict__ off,
-                                                       typename Traits<T>::Stream *__restrict__ planes,
-                                                       uint64_t stride) {
+template <class T>
+__global__ __launch_bounds__(BlkStage<T>::kTPB) void k_gb_write(const T *__restrict__ x, uint32_t n, int32_t exp,
+                                                             const KeyOf<T> *key, int nb, int compat,
+                                                             const uint32_t *__restrict__ off,
+                                                             typename Traits<T>::Stream *__restrict__ planes,
+                                                             uint64_t stride) {
     using F = typename Traits<T>::Fixed;
     using S = typename Traits<T>::Stream;
-    constexpr int B = Traits<T>::kBlock;
-    __shared__ uint32_t tot[kWavesPerBlock][64];
-    const uint32_t lane = lane_id();
-    const uint32_t wv = threadIdx.x / kWave;
-    const uint64_t wave = (uint64_t)blockIdx.x * kWavesPerBlock + wv;
-    const uint64_t base = wave * kChunk;
-    const uint32_t nblk = (n - 1) / B + 1;
+    constexpr int TPB = BlkStage<T>::kTPB, B = BlkStage<T>::kB, PI = BlkStage<T>::kPitch;
+    __shared__ T sx[TPB * PI];
+    __shared__ uint32_t wtot[TPB / 64];
+    const uint64_t b0 = (uint64_t)blockIdx.x * TPB;
+    stage_in<T>(x, n, b0 * B, sx);
+    __syncthreads();
     const int shift = nb - level_exp<T>(exp, key);
-    uint64_t w[MAXNB];
-    uint64_t sg = 0;
+    const uint64_t b = b0 + threadIdx.x;
+    const uint32_t nblk = (n - 1) / B + 1;
+    const T *row = sx + threadIdx.x * PI;
+    S sg = 0;
+    F orv = 0;
+    int r;
+    if constexpr (sizeof(T) == 4) {
+        uint32_t m[32];
 #pragma unroll
-    for (int p = 0; p < MAXNB; p++) w[p] = 0;
-    if (base < n) {
-        for (int c = 0; c < 64; c++) {
-            const uint64_t e = base + (uint64_t)c * 64 + lane;
-            const T xv = e < n ? x[e] : (T)0;
-            const F m = gb_mag<T>(xv, shift, compat);
-            const uint64_t sm = ballot64(xv < 0);
-            sg = (lane == (uint32_t)c) ? sm : sg;
+        for (int j = 0; j < 32; j++) {
+            m[j] = gb_mag<T>(row[j], shift, compat);
+            orv |= m[j];
+            sg |= (S)(row[j] < 0) << j;
+        }
+        r = b < nblk ? rb_of<F>(orv, nb) : 255;
+        transpose32(m);
 #pragma unroll
-            for (int p = 0; p < MAXNB; p++) {
-                if (p < nb) {
-                    const uint64_t bm = ballot64((m >> (nb - 1 - p)) & 1u);
-                    w[p] = (lane == (uint32_t)c) ? bm : w[p];
-                }
+        for (int k = 31; k >= 0; k--) {
+            const int p = nb - 1 - k;  // plane p holds bit nb - 1 - p
+            if (p >= 0) {
+                const uint32_t pre = wg_prefix<TPB>(r <= p, r == p, wtot);
+                uint64_t pos = off[(uint64_t)blockIdx.x * nb + p] + pre;
+                S *dst = planes + (uint64_t)p * stride;
+                if (r == p) dst[pos++] = sg;
+                if (r <= p) dst[pos] = m[k];
             }
         }
-    }
-    // this lane owns blocks b0 (and b1 for u32 streams) of group `lane`
-    const uint64_t g = base / 64 + lane;
-    const uint64_t b0 = g * (64 / B);
-    const bool v0 = b0 < nblk, v1 = (B == 32) && (b0 + 1 < nblk);
-    int r0 = nb, r1 = nb;
+    } else {
+        uint32_t lo[64], hi[64];
 #pragma unroll
-    for (int p = MAXNB - 1; p >= 0; p--) {
-        if (p < nb) {
-            const uint64_t lo = (B == 32) ? (w[p] & 0xffffffffull) : w[p];
-            const uint64_t hi = (B == 32) ? (w[p] >> 32) : 0;
-            if (lo) r0 = p;
-            if (hi) r1 = p;
+        for (int j = 0; j < 64; j++) {
+            const uint64_t q = gb_mag<T>(row[j], shift, compat);
+            orv |= q;
+            lo[j] = (uint32_t)q;
+            hi[j] = (uint32_t)(q >> 32);
+            sg |= (S)(row[j] < 0) << j;
         }
-    }
-    if (!v0) r0 = 255;
-    if (!v1) r1 = 255;
-    // per-plane word counts and in-wave prefix (ballot + mbcnt)
-#pragma unroll
-    for (int p = 0; p < MAXNB; p++) {
-        if (p < nb) {
-            const uint64_t a = ballot64(r0 == p), bq = ballot64(r0 <= p), c2 = ballot64(r1 == p), d = ballot64(r1 <= p);
-            if (lane == 0)
-                tot[wv][p] = __popcll(a) + __popcll(bq) + __popcll(c2) + __popcll(d);
-        }
-    }
-    __syncthreads();
-    if (base >= n) return;
-#pragma unroll
-    for (int p = 0; p < MAXNB; p++) {
-        if (p < nb) {
-            const uint64_t a = ballot64(r0 == p), bq = ballot64(r0 <= p), c2 = ballot64(r1 == p), d = ballot64(r1 <= p);
-            uint64_t pos = off[(uint64_t)blockIdx.x * nb + p];
-            for (uint32_t q = 0; q < wv; q++) pos += tot[q][p];
-            pos += lanes_below(a) + lanes_below(bq) + lanes_below(c2) + lanes_below(d);
-            S *dst = planes + (uint64_t)p * stride;
-            if (r0 == p) dst[pos++] = (S)((B == 32) ? (sg & 0xffffffffull) : sg);
-            if (r0 <= p) dst[pos++] = (S)((B == 32) ? (w[p] & 0xffffffffull) : w[p]);
-            if (r1 == p) dst[pos++] = (S)(sg >> 32);
-            if (r1 <= p) dst[pos++] = (S)(w[p] >> 32);
-        }
+        r = b < nblk ? rb_of<F>(orv, nb) : 255;
+        transpose64(lo, hi);
+        static_for<0, 64>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            const int p = nb - 1 - k;
+            if (p >= 0) {
+                const uint32_t pre = wg_prefix<TPB>(r <= p, r == p, wtot);
+                uint64_t pos = off[(uint64_t)blockIdx.x * nb + p] + pre;
+                S *dst = planes + (uint64_t)p * stride;
+                if (r == p) dst[pos++] = sg;
+                if (r <= p) dst[pos] = word64_get<k>(lo, hi);
+            }
+        });
     }
 }
 
+template <class T> static uint32_t gb_nwg(uint32_t n) {
+    const uint32_t nblk = (n - 1) / Traits<T>::kBlock + 1;
+    return div_up(nblk, BlkStage<T>::kTPB);
+}
+
 template <class T> size_t gb_workspace_bytes(uint32_t n, int nb) {
-    const uint32_t nwg = div_up(div_up(n, kChunk), kWavesPerBlock);
+    const uint32_t nwg = gb_nwg<T>(n);
     return (size_t)nwg * (nb + 1) * 4 + (size_t)nwg * nb * 4 + 256;
 }
 template size_t gb_workspace_bytes<float>(uint32_t, int);
@@ -433,19 +520,15 @@ int launch_gb_encode(const T *x, uint32_t n, int32_t exp, const KeyOf<T> *key, i
     const uint32_t nblk = (n - 1) / B + 1;
     if (stride < 2ull * nblk) { set_error("grouped encode: plane stride below 2 * nblocks"); return kErrArg; }
     if (ws_bytes < gb_workspace_bytes<T>(n, nb)) { set_error("grouped encode: workspace too small"); return kErrArg; }
-    const uint32_t nwg = div_up(div_up(n, kChunk), kWavesPerBlock);
+    const uint32_t nwg = gb_nwg<T>(n);
     uint32_t *hist = static_cast<uint32_t *>(ws);
     uint32_t *off = hist + (size_t)nwg * (nb + 1);
-    hipLaunchKernelGGL(k_gb_rb<T>, dim3(nwg), dim3(kThreads), 0, s, x, n, exp, key, nb, compat, rb, hist);
+    hipLaunchKernelGGL(k_gb_rb<T>, dim3(nwg), dim3(BlkStage<T>::kTPB), 0, s, x, n, exp, key, nb, compat, rb, hist);
     PRODM_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_gb_scan, dim3(nb), dim3(1024), 0, s, hist, nwg, nb + 1, 0, nb, off, words_per_plane);
     PRODM_CHECK_LAUNCH();
-    if (nb <= 32)
-        hipLaunchKernelGGL((k_gb_write<T, 32>), dim3(nwg), dim3(kThreads), 0, s, x, n, exp, key, nb, compat, off,
-                           planes, stride);
-    else
-        hipLaunchKernelGGL((k_gb_write<T, 64>), dim3(nwg), dim3(kThreads), 0, s, x, n, exp, key, nb, compat, off,
-                           planes, stride);
+    hipLaunchKernelGGL(k_gb_write<T>, dim3(nwg), dim3(BlkStage<T>::kTPB), 0, s, x, n, exp, key, nb, compat, off, planes,
+                       stride);
     PRODM_CHECK_LAUNCH();
     return kOk;
 }
@@ -455,15 +538,16 @@ template int launch_gb_encode<double>(const double *, uint32_t, int32_t, const u
                                       uint8_t *, uint64_t *, uint64_t, uint32_t *, void *, size_t, hipStream_t);
 
 // ---- GroupedBP progressive decode ----------------------------------------------
-// rb histogram per workgroup (decode side; same chunking as the encoder)
+// rb histogram per workgroup (same block ranges as the encoder)
 template <class T>
-__global__ __launch_bounds__(kThreads) void k_gb_rbhist(const uint8_t *__restrict__ rb, uint32_t nblk, int nbins,
-                                                        uint32_t *__restrict__ hist) {
+__global__ __launch_bounds__(256) void k_gb_rbhist(const uint8_t *__restrict__ rb, uint32_t nblk, int nbins,
+                                                   uint32_t *__restrict__ hist) {
+    constexpr int TPB = BlkStage<T>::kTPB;
     __shared__ uint32_t h[256];
-    for (int i = threadIdx.x; i < nbins; i += kThreads) h[i] = 0;
+    for (int i = threadIdx.x; i < nbins; i += 256) h[i] = 0;
     __syncthreads();
-    const uint64_t b0 = (uint64_t)blockIdx.x * blocks_per_wg<T>();
-    for (uint32_t i = threadIdx.x; i < blocks_per_wg<T>(); i += kThreads) {
+    const uint64_t b0 = (uint64_t)blockIdx.x * TPB;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)TPB; i += 256) {
         const uint64_t b = b0 + i;
         if (b < nblk) {
             const int r = rb[b];
@@ -471,7 +555,7 @@ __global__ __launch_bounds__(kThreads) void k_gb_rbhist(const uint8_t *__restric
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < nbins; i += kThreads) hist[(uint64_t)blockIdx.x * nbins + i] = h[i];
+    for (int i = threadIdx.x; i < nbins; i += 256) hist[(uint64_t)blockIdx.x * nbins + i] = h[i];
 }
 
 template <class S> __device__ inline S load_word(const uint8_t *base, uint64_t byte_off, bool aligned) {
@@ -482,81 +566,73 @@ template <class S> __device__ inline S load_word(const uint8_t *base, uint64_t b
     return v;
 }
 
-template <class T, int MAXK>
-__global__ __launch_bounds__(kThreads) void k_gb_decode(PlanePtrs pl, uint32_t n, int32_t exp, const KeyOf<T> *key,
-                                                        int start, int k, const uint8_t *__restrict__ rb,
-                                                        typename Traits<T>::Stream *__restrict__ signs,
-                                                        const uint32_t *__restrict__ off, T *__restrict__ out,
-                                                        uint64_t hdr0) {
-    using F = typename Traits<T>::Fixed;
+template <class T>
+__global__ __launch_bounds__(BlkStage<T>::kTPB) void k_gb_decode(PlanePtrs pl, uint32_t n, int32_t exp,
+                                                              const KeyOf<T> *key, int start, int k,
+                                                              const uint8_t *__restrict__ rb,
+                                                              typename Traits<T>::Stream *__restrict__ signs,
+                                                              const uint32_t *__restrict__ off, T *__restrict__ out,
+                                                              uint64_t hdr0) {
     using S = typename Traits<T>::Stream;
-    constexpr int B = Traits<T>::kBlock;
-    __shared__ uint32_t tot[kWavesPerBlock][64];
-    const uint32_t lane = lane_id();
-    const uint32_t wv = threadIdx.x / kWave;
-    const uint64_t wave = (uint64_t)blockIdx.x * kWavesPerBlock + wv;
-    const uint64_t base = wave * kChunk;
+    constexpr int TPB = BlkStage<T>::kTPB, B = BlkStage<T>::kB, PI = BlkStage<T>::kPitch;
+    __shared__ T sx[TPB * PI];
+    __shared__ uint32_t wtot[TPB / 64];
+    const uint64_t b0 = (uint64_t)blockIdx.x * TPB;
+    const uint64_t b = b0 + threadIdx.x;
     const uint32_t nblk = (n - 1) / B + 1;
     const int e0 = level_exp<T>(exp, key);
     const int end = start + k;
-    const uint64_t g = base / 64 + lane;
-    const uint64_t b0 = g * (64 / B);
-    const bool v0 = b0 < nblk, v1 = (B == 32) && (b0 + 1 < nblk);
-    const int r0 = v0 ? rb[b0] : 255, r1 = v1 ? rb[b0 + 1] : 255;
-    for (int i = 0; i < k; i++) {
-        const int p = start + i;
-        const uint64_t a = ballot64(r0 == p), bq = ballot64(r0 <= p), c2 = ballot64(r1 == p), d = ballot64(r1 <= p);
-        if (lane == 0) tot[wv][i] = __popcll(a) + __popcll(bq) + __popcll(c2) + __popcll(d);
+    const bool valid = b < nblk;
+    const int r = valid ? rb[b] : 255;
+    S sg = valid ? signs[b] : 0;
+    // acc bit (end - 1 - p) comes from plane p; planes below rb are zero
+    constexpr int W = 8 * sizeof(S);
+    uint32_t lo[W], hi[sizeof(T) == 8 ? W : 1];
+#pragma unroll
+    for (int q = 0; q < W; q++) {
+        lo[q] = 0;
+        if constexpr (sizeof(T) == 8) hi[q] = 0;
     }
-    __syncthreads();
-    if (base >= n) return;
-    S s0 = v0 ? signs[b0] : 0, s1 = v1 ? signs[b0 + 1] : 0;
-    uint64_t wl[MAXK];
-#pragma unroll
-    for (int i = 0; i < MAXK; i++) wl[i] = 0;
-#pragma unroll
-    for (int i = 0; i < MAXK; i++) {
-        if (i < k) {
-            const int p = start + i;
-            const uint64_t a = ballot64(r0 == p), bq = ballot64(r0 <= p), c2 = ballot64(r1 == p), d = ballot64(r1 <= p);
-            uint64_t pos = off[(uint64_t)blockIdx.x * k + i];
-            for (uint32_t q = 0; q < wv; q++) pos += tot[q][i];
-            pos += lanes_below(a) + lanes_below(bq) + lanes_below(c2) + lanes_below(d);
+    static_for<0, W>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const int p = end - 1 - q;  // acc bit q <- plane p
+        if (p >= start) {
+            const int i = p - start;
+            const uint32_t pre = wg_prefix<TPB>(r <= p, r == p, wtot);
+            uint64_t pos = off[(uint64_t)blockIdx.x * k + i] + pre;
             const uint8_t *sb = static_cast<const uint8_t *>(pl.p[i]);
             const uint64_t hb = (i == 0) ? hdr0 : 0;
             const bool al = (hb % sizeof(S)) == 0;
-            uint64_t lo = 0, hi = 0;
-            if (r0 == p) { s0 = load_word<S>(sb, hb + pos * sizeof(S), al); pos++; }
-            if (r0 <= p) { lo = load_word<S>(sb, hb + pos * sizeof(S), al); pos++; }
-            if (r1 == p) { s1 = load_word<S>(sb, hb + pos * sizeof(S), al); pos++; }
-            if (r1 <= p) { hi = load_word<S>(sb, hb + pos * sizeof(S), al); pos++; }
-            wl[i] = (B == 32) ? ((hi << 32) | (lo & 0xffffffffull)) : lo;
-        }
-    }
-    // persist the signs of blocks recorded in this window
-    if (v0 && r0 >= start && r0 < end) signs[b0] = s0;
-    if (v1 && r1 >= start && r1 < end) signs[b0 + 1] = s1;
-    const uint64_t sg = (B == 32) ? (((uint64_t)s1 << 32) | ((uint64_t)s0 & 0xffffffffull)) : (uint64_t)s0;
-    // transpose: group c's words live in lane c
-    for (int c = 0; c < 64; c++) {
-        const uint64_t e = base + (uint64_t)c * 64 + lane;
-        if (base + (uint64_t)c * 64 >= n) break;
-        F acc = 0;
-#pragma unroll
-        for (int i = 0; i < MAXK; i++) {
-            if (i < k) {
-                const uint64_t word = readlane_u64(wl[i], c);
-                acc |= (F)((word >> lane) & 1u) << (k - 1 - i);
+            if (r == p) { sg = load_word<S>(sb, hb + pos * sizeof(S), al); pos++; }
+            if (r <= p) {
+                const S w = load_word<S>(sb, hb + pos * sizeof(S), al);
+                if constexpr (sizeof(T) == 4) lo[q] = (uint32_t)w;
+                else word64_set<q>(lo, hi, (uint64_t)w);
             }
         }
-        const uint64_t sgc = readlane_u64(sg, c);
-        const int rr0 = __builtin_amdgcn_readlane(r0, c), rr1 = __builtin_amdgcn_readlane(r1, c);
-        const int rmine = (B == 32 && lane >= 32) ? rr1 : rr0;
-        T y = dev_ldexp((T)acc, e0 - end);
-        if ((sgc >> lane) & 1u) y = -y;
-        if (rmine >= end) y = (T)0;
-        if (e < n) out[e] = y;
+    });
+    // persist the signs of blocks recorded in this window
+    if (valid && r >= start && r < end) signs[b] = sg;
+    T *row = sx + threadIdx.x * PI;
+    if constexpr (sizeof(T) == 4) {
+        transpose32(lo);
+#pragma unroll
+        for (int j = 0; j < 32; j++) {
+            T y = dev_ldexp((T)lo[j], e0 - end);
+            if ((sg >> j) & 1u) y = -y;
+            row[j] = (r >= end) ? (T)0 : y;
+        }
+    } else {
+        transpose64(lo, hi);
+#pragma unroll
+        for (int j = 0; j < 64; j++) {
+            T y = dev_ldexp((T)(((uint64_t)hi[j] << 32) | lo[j]), e0 - end);
+            if ((sg >> j) & 1u) y = -y;
+            row[j] = (r >= end) ? (T)0 : y;
+        }
     }
+    __syncthreads();
+    stage_out<T>(out, n, b0 * B, sx);
 }
 
 template <class T>
@@ -571,22 +647,21 @@ int launch_gb_decode(const PlanePtrs &streams, uint32_t n, int32_t exp, const Ke
         else return launch_fill_u64(reinterpret_cast<uint64_t *>(out), n, 0ull, s);
     }
     const uint32_t nblk = (n - 1) / B + 1;
-    const uint32_t nwg = div_up(nblk, blocks_per_wg<T>());
+    const uint32_t nwg = gb_nwg<T>(n);
     const int nbins = start + k;  // rb values >= end never contribute
-    if (ws_bytes < (size_t)nwg * (nbins + k) * 4) { set_error("grouped decode: workspace too small"); return kErrArg; }
+    if (ws_bytes < (size_t)nwg * (nbins + k) * 4 + (size_t)k * 4) {
+        set_error("grouped decode: workspace too small");
+        return kErrArg;
+    }
     uint32_t *hist = static_cast<uint32_t *>(ws);
     uint32_t *off = hist + (size_t)nwg * nbins;
     uint32_t *totals = off + (size_t)nwg * k;
-    hipLaunchKernelGGL(k_gb_rbhist<T>, dim3(nwg), dim3(kThreads), 0, s, rb, nblk, nbins, hist);
+    hipLaunchKernelGGL(k_gb_rbhist<T>, dim3(nwg), dim3(256), 0, s, rb, nblk, nbins, hist);
     PRODM_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_gb_scan, dim3(k), dim3(1024), 0, s, hist, nwg, nbins, start, k, off, totals);
     PRODM_CHECK_LAUNCH();
-    if (k <= 32)
-        hipLaunchKernelGGL((k_gb_decode<T, 32>), dim3(nwg), dim3(kThreads), 0, s, streams, n, exp, key, start, k, rb,
-                           signs, off, out, hdr0);
-    else
-        hipLaunchKernelGGL((k_gb_decode<T, 64>), dim3(nwg), dim3(kThreads), 0, s, streams, n, exp, key, start, k, rb,
-                           signs, off, out, hdr0);
+    hipLaunchKernelGGL(k_gb_decode<T>, dim3(nwg), dim3(BlkStage<T>::kTPB), 0, s, streams, n, exp, key, start, k, rb, signs,
+                       off, out, hdr0);
     PRODM_CHECK_LAUNCH();
     return kOk;
 }
@@ -596,9 +671,7 @@ template int launch_gb_decode<double>(const PlanePtrs &, uint32_t, int32_t, cons
                                       const uint8_t *, uint64_t *, double *, void *, size_t, uint64_t, hipStream_t);
 
 template <class T> size_t gb_decode_workspace_bytes(uint32_t n, int start, int k) {
-    constexpr int B = Traits<T>::kBlock;
-    const uint32_t nblk = (n - 1) / B + 1;
-    const uint32_t nwg = div_up(nblk, blocks_per_wg<T>());
+    const uint32_t nwg = gb_nwg<T>(n);
     return (size_t)nwg * (start + 2 * k) * 4 + 64 * 4 + 256;
 }
 template size_t gb_decode_workspace_bytes<float>(uint32_t, int, int);

@@ -121,6 +121,20 @@ static int recompose_impl(const T *in, T *out, const uint32_t *dims, int nd, int
 }
 
 template <class T>
+static int adjoint_impl(const T *g_in, T *out, const uint32_t *dims, int nd, int L, const uint32_t *strides, void *ws,
+                        size_t ws_bytes, hipStream_t s) {
+    const LevelGeom g = make_geom(dims, nd, L);
+    if (int rc = check_geom(g)) return rc;
+    if (!g_in || !out || (const void *)g_in == (const void *)out) {
+        set_error("recompose_adjoint: need distinct device input and output");
+        return kErrArg;
+    }
+    uint64_t st[3];
+    strides3(g, strides, st);
+    return mgard_recompose_adjoint<T>(g_in, st, g, out, st, ws, ws_bytes, s);
+}
+
+template <class T>
 static int interleave_impl(const T *data, const uint32_t *dims, int nd, const uint32_t *fine, const uint32_t *coarse,
                            const uint32_t *strides, T *buf, bool repos, hipStream_t s) {
     if (nd < 1 || nd > 3 || !dims || !fine || !coarse) { set_error("interleave: bad arguments"); return kErrArg; }
@@ -293,7 +307,7 @@ template <class T> static int plan_refactor(prodm_mdr_plan *p, const T *in, hipS
     const uint64_t st[3] = {(uint64_t)g.dims[1] * g.dims[2], g.dims[2], 1};
     StageTimer *tm = p->timer.on ? &p->timer : nullptr;
     if (tm) tm->stream = s;
-    rc = mgard_decompose<T>(in, st, g, sink, keys, p->ws, p->ws_bytes, s, tm);
+    rc = mgard_decompose<T>(in, st, g, sink, keys, p->ws, p->ws_bytes, s, tm, true);
     if (rc) return rc;
     for (int l = g.L; l >= 0; l--) {
         StageScope sc(tm, l == g.L ? kStEncTop : kStEncRest);
@@ -335,7 +349,7 @@ template <class T> static int plan_retrieve(prodm_mdr_plan *p, const uint8_t *k,
         if (rc) return rc;
     }
     const uint64_t st[3] = {(uint64_t)g.dims[1] * g.dims[2], g.dims[2], 1};
-    return mgard_recompose<T>(src, g, 0, out, st, p->ws, p->ws_bytes, s, tm);
+    return mgard_recompose<T>(src, g, 1, out, st, p->ws, p->ws_bytes, s, tm);
 }
 
 }  // namespace prodm
@@ -354,6 +368,19 @@ int prodm_level_geometry(const uint32_t *dims, int ndims, int target_level, uint
         if (out_elements) out_elements[l] = g.elems[l];
     }
     return g.ok ? 1 : 0;
+}
+
+size_t prodm_recompose_adjoint_workspace_bytes(const uint32_t *dims, int ndims, int target_level) {
+    const LevelGeom g = make_geom(dims, ndims, target_level);
+    return g.ok ? adjoint_workspace_bytes(g) : 0;
+}
+int prodm_recompose_adjoint_f32(const float *d_g, float *d_out, const uint32_t *dims, int ndims, int target_level,
+                                const uint32_t *strides, void *d_ws, size_t ws_bytes, prodm_stream_t stream) {
+    return adjoint_impl<float>(d_g, d_out, dims, ndims, target_level, strides, d_ws, ws_bytes, (hipStream_t)stream);
+}
+int prodm_recompose_adjoint_f64(const double *d_g, double *d_out, const uint32_t *dims, int ndims, int target_level,
+                                const uint32_t *strides, void *d_ws, size_t ws_bytes, prodm_stream_t stream) {
+    return adjoint_impl<double>(d_g, d_out, dims, ndims, target_level, strides, d_ws, ws_bytes, (hipStream_t)stream);
 }
 
 size_t prodm_mgard_workspace_bytes(const uint32_t *dims, int ndims, int target_level, int dtype) {
@@ -526,6 +553,7 @@ int prodm_mdr_plan_create(prodm_mdr_plan **out, const uint32_t *dims, int ndims,
     int rc = kOk;
     p->ws_bytes = mgard_workspace_bytes(g, p->elem);
     if (!rc) rc = plan_alloc(&p->ws, p->ws_bytes, p->device_bytes);
+    if (!rc) rc = mgard_upload_tables(g, p->elem, p->ws, p->ws_bytes);
     if (!rc) rc = plan_alloc(&p->levels, lo * p->elem, p->device_bytes);
     if (!rc) rc = plan_alloc(&p->keys, 8 * (g.L + 1), p->device_bytes);
     if (!rc) rc = plan_alloc(&p->planes, po * sw, p->device_bytes);

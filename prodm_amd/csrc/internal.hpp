@@ -95,13 +95,25 @@ template <class T> struct CoefSink {
     uint64_t s[3] = {0, 0, 1};   // strides of the full array
     T *level[33] = {nullptr};    // interleaved mode: level buffers (0..L)
 };
+// Thomas factor tables live in the workspace. tables_ready: they were
+// uploaded once by mgard_upload_tables (synchronous); otherwise each call
+// uploads them on the stream.
+int mgard_upload_tables(const LevelGeom &g, size_t elem, void *ws, size_t ws_bytes);
 template <class T>
 int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g, const CoefSink<T> &sink,
-                    KeyOf<T> *level_keys, void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm = nullptr);
+                    KeyOf<T> *level_keys, void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm = nullptr,
+                    bool tables_ready = false);
 template <class T>
-int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int unused, T *out, const uint64_t *out_strides,
+int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int tables_ready, T *out, const uint64_t *out_strides,
                     void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm = nullptr);
-int mgard_recompose_adjoint(const double *g_in, const uint64_t *in_strides, const LevelGeom &g, double *out,
+// Transpose of recompose (adjoint.hip): g_in = dJ/du on the level-L box
+// (strides in_strides), out = dJ/dcoefficients in the decompose layout.
+size_t adjoint_workspace_bytes(const LevelGeom &g);
+template <class T>
+int mgard_recompose_adjoint(const T *g_in, const uint64_t *in_strides, const LevelGeom &g, T *out,
                             const uint64_t *out_strides, void *ws, size_t ws_bytes, hipStream_t s);
+int mgard_adjoint_coarse(const double *gfine, const LevelGeom &g, int l, double *h_out, void *ws, size_t ws_bytes,
+                         hipStream_t s);
+const double *mgard_adjoint_solution(const LevelGeom &g, void *ws);
 
 }  // namespace prodm

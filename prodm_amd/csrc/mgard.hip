@@ -26,8 +26,6 @@
 
 namespace prodm {
 
-constexpr int kTJ0 = 8, kTJ1 = 8, kTJ2 = 32;          // coarse tile per workgroup
-constexpr int kR1 = 2 * kTJ1 + 3, kR2 = 2 * kTJ2 + 3;  // fine region (with halo)
 constexpr int kLoadThreads = 256;
 
 enum LoadMode : int { kDecompose = 0, kRecompose = 1, kAdjointPT = 2 };
@@ -57,6 +55,7 @@ template <class T> struct LoadParams {
     T *uc;                 // decompose: coarse nodal values (dense c)
     double *f;             // load vector (dense c)
     KeyOf<T> *key;         // decompose: level max |W|
+    int march;             // coarse planes per workgroup along dim 0
 };
 
 template <class T>
@@ -65,36 +64,18 @@ __device__ inline uint64_t coef_index_full(const LoadParams<T> &P, uint32_t i0, 
 }
 template <class T>
 __device__ inline uint64_t coef_index_ilv(const LoadParams<T> &P, uint32_t i0, uint32_t i1, uint32_t i2) {
-    return ilv_index(rpos(i0, P.c[0]), rpos(i1, P.c[1]), rpos(i2, P.c[2]), P.n, P.ilv_c);
-}
-
-// Multilinear interpolant of fine node (r1, r2) of the current plane from
-// its coarse neighbours; corners lexicographic over the odd dims, pairwise
-// tree sum, then * 1/2, 1/4, 1/8 (same expression as the CPU restatement).
-template <class T>
-__device__ inline T interp_lds(const T (*A)[kR2], const T (*Bp)[kR2], const T (*Cp)[kR2], int o0, int r1, int r2) {
-    // A: plane i0-1, Bp: plane i0+1 (used when o0), Cp: current plane
-    const int o1 = r1 & 1, o2 = r2 & 1;
-    auto V = [&](int x0, int x1, int x2) -> T {
-        const int a1 = o1 ? r1 + 2 * x1 - 1 : r1;
-        const int a2 = o2 ? r2 + 2 * x2 - 1 : r2;
-        const T(*pl)[kR2] = o0 ? (x0 ? Bp : A) : Cp;
-        return pl[a1][a2];
-    };
-    const int cnt = o0 + o1 + o2;
-    if (cnt == 1) {
-        if (o2) return (V(0, 0, 0) + V(0, 0, 1)) * (T)0.5;
-        if (o1) return (V(0, 0, 0) + V(0, 1, 0)) * (T)0.5;
-        return (V(0, 0, 0) + V(1, 0, 0)) * (T)0.5;
+    // ilv_index (DirectInterleaver order) with scalar operands: taking the
+    // address of the by-value kernel argument would force a scratch copy
+    const uint32_t p0 = rpos(i0, P.c[0]), p1 = rpos(i1, P.c[1]), p2 = rpos(i2, P.c[2]);
+    const uint32_t n1 = P.n[1], n2 = P.n[2], c0 = P.ilv_c[0], c1 = P.ilv_c[1], c2 = P.ilv_c[2];
+    const uint64_t n12 = (uint64_t)n1 * n2;
+    const uint64_t plane_in = n12 - (uint64_t)c1 * c2;
+    if (p0 < c0) {
+        const uint64_t base = (uint64_t)p0 * plane_in;
+        if (p1 < c1) return base + (uint64_t)p1 * (n2 - c2) + (p2 - c2);
+        return base + (uint64_t)c1 * (n2 - c2) + (uint64_t)(p1 - c1) * n2 + p2;
     }
-    if (cnt == 2) {
-        if (!o0) return ((V(0, 0, 0) + V(0, 0, 1)) + (V(0, 1, 0) + V(0, 1, 1))) * (T)0.25;
-        if (!o1) return ((V(0, 0, 0) + V(0, 0, 1)) + (V(1, 0, 0) + V(1, 0, 1))) * (T)0.25;
-        return ((V(0, 0, 0) + V(0, 1, 0)) + (V(1, 0, 0) + V(1, 1, 0))) * (T)0.25;
-    }
-    return (((V(0, 0, 0) + V(0, 0, 1)) + (V(0, 1, 0) + V(0, 1, 1))) +
-            ((V(1, 0, 0) + V(1, 0, 1)) + (V(1, 1, 0) + V(1, 1, 1)))) *
-           (T)0.125;
+    return (uint64_t)c0 * plane_in + (uint64_t)(p0 - c0) * n12 + (uint64_t)p1 * n2 + p2;
 }
 
 struct DimTile {
@@ -110,190 +91,322 @@ __device__ inline DimTile dim_tile(uint32_t tile, int TJ, uint32_t c, uint32_t n
     return t;
 }
 
+// One workgroup per coarse tile: kM1 rows (dim 1) x kM2 columns (dim 2),
+// marching along dim 0 over kM0 coarse planes. Each fine plane of the tile
+// plus a 2-node halo (kN1 x kN2, compile-time) lives in LDS; the next pair of
+// planes is prefetched into registers while the current pair is processed.
+// Per fine plane: W (in LDS), coefficient / coarse-value / max stores, the
+// load stencil along dim 2 (LDS, FP64) and dim 1 (registers, one coarse
+// column per thread), and the dim-0 stencil as a running sum over planes.
+constexpr int kM1 = 8, kM2 = 32;
+constexpr int kN1 = 2 * kM1 + 3, kN2 = 2 * kM2 + 3, kPL = kN1 * kN2;
+constexpr int kPER = (kPL + kLoadThreads - 1) / kLoadThreads;   // plane elements per thread
+constexpr int kG2 = kN1 * kM2;
+constexpr int kG2PER = (kG2 + kLoadThreads - 1) / kLoadThreads;
+
+__device__ inline uint32_t xcd_remap(uint32_t bid, uint32_t nblocks) {
+    // workgroups are dispatched round-robin over the 8 XCDs: give XCD x the
+    // contiguous range [x q, (x + 1) q) of the first 8 q tiles (a bijection;
+    // the remaining tail keeps its index)
+    const uint32_t q = nblocks / 8;
+    if (bid >= 8 * q) return bid;
+    return (bid % 8) * q + bid / 8;
+}
+
+struct MarchTile {
+    int js0, tj0, js1, tj1, js2, tj2;  // coarse tile
+    int rs1, rs2;                      // fine region start (dims 1, 2)
+};
+__device__ inline MarchTile march_tile(uint32_t bt, uint32_t n0, uint32_t n1, uint32_t n2, uint32_t c0, uint32_t c1,
+                                       uint32_t c2, int kM0) {
+    const uint32_t n[3] = {n0, n1, n2}, c[3] = {c0, c1, c2};
+    const uint32_t t2n = (n[2] == 1) ? 1 : (c[2] + kM2 - 1) / kM2;
+    const uint32_t t1n = (n[1] == 1) ? 1 : (c[1] + kM1 - 1) / kM1;
+    const uint32_t a2 = bt % t2n, a1 = (bt / t2n) % t1n, a0 = bt / (t2n * t1n);
+    MarchTile t;
+    t.js2 = (n[2] == 1) ? 0 : (int)a2 * kM2;
+    t.tj2 = (n[2] == 1) ? 1 : min(kM2, (int)c[2] - t.js2);
+    t.rs2 = (n[2] == 1) ? 0 : 2 * t.js2 - 2;
+    t.js1 = (n[1] == 1) ? 0 : (int)a1 * kM1;
+    t.tj1 = (n[1] == 1) ? 1 : min(kM1, (int)c[1] - t.js1);
+    t.rs1 = (n[1] == 1) ? 0 : 2 * t.js1 - 2;
+    t.js0 = (n[0] == 1) ? 0 : (int)a0 * kM0;
+    t.tj0 = (n[0] == 1) ? 1 : min(kM0, (int)c[0] - t.js0);
+    return t;
+}
+
 template <class T, int MODE, bool ILV>
 __global__ __launch_bounds__(kLoadThreads) void k_load(LoadParams<T> P) {
     using K = KeyOf<T>;
-    __shared__ T Ue0[kR1][kR2], Ue1[kR1][kR2], Uo[kR1][kR2], Wb[kR1][kR2];
-    __shared__ double g2[kR1][kTJ2];
+    constexpr bool kDec = MODE == kDecompose;
+    constexpr int kPE = kDec ? kPL : 1;  // input planes staged only by decompose
+    __shared__ T Ea[kPE], Eb[kPE], O[kPE], Wb[kPL];
+    __shared__ double G2[kG2];
     const int tid = threadIdx.x;
-    const uint32_t t2n = (P.n[2] == 1) ? 1 : (P.c[2] + kTJ2 - 1) / kTJ2;
-    const uint32_t t1n = (P.n[1] == 1) ? 1 : (P.c[1] + kTJ1 - 1) / kTJ1;
-    const uint32_t bt = blockIdx.x;
-    const DimTile D2 = dim_tile(bt % t2n, kTJ2, P.c[2], P.n[2]);
-    const DimTile D1 = dim_tile((bt / t2n) % t1n, kTJ1, P.c[1], P.n[1]);
-    const DimTile D0 = dim_tile(bt / (t2n * t1n), kTJ0, P.c[0], P.n[0]);
-    const int R1 = D1.R, R2 = D2.R, RR = R1 * R2;
-    // ownership of fine indices: [2js - 1, 2(js + tj) - 1) clipped
-    auto own = [](const DimTile &d, int i, uint32_t n) -> bool {
-        if (n == 1) return true;
-        return i >= 2 * d.js - 1 && i < 2 * (d.js + d.tj) - 1 && i >= 0 && i < (int)n;
+    const uint32_t n0 = P.n[0], n1 = P.n[1], n2 = P.n[2], c0 = P.c[0], c1 = P.c[1], c2 = P.c[2];
+    const uint32_t t2n = (n2 == 1) ? 1 : (c2 + kM2 - 1) / kM2;
+    const uint32_t t1n = (n1 == 1) ? 1 : (c1 + kM1 - 1) / kM1;
+    const uint32_t t0n = (n0 == 1) ? 1 : (c0 + P.march - 1) / P.march;
+    const MarchTile M = march_tile(xcd_remap(blockIdx.x, t0n * t1n * t2n), n0, n1, n2, c0, c1, c2, P.march);
+    const bool act0 = n0 > 1, act1 = n1 > 1, act2 = n2 > 1;
+
+    // ---- per-slot constants: slot m of this thread is plane element
+    // idx = tid + 256 m at (r1, r2) of the (kN1 x kN2) region; everything
+    // that depends only on (i1, i2) is computed once here, so that a plane
+    // step costs a scalar plane offset plus one add per access.
+    // offsets are in bytes and 32-bit, so that every access is a scalar
+    // plane pointer plus one 32-bit lane offset
+    uint32_t in_off[kPER];  // decompose / adjoint: byte offset in an input plane
+    uint32_t offA[kPER];    // coefficient byte offset in an even plane (coarse slots: u_c offset)
+    uint32_t offB[kPER];    // coefficient byte offset in an odd plane
+    int qb[kPER];           // LDS offset of the low interpolation corner
+    uint32_t bits = 0;      // 4 bits per slot: valid, odd i1, odd i2, owned (i1, i2)
+    const uint32_t n12 = n1 * n2, plane_in = n12 - c1 * c2;
+#pragma unroll
+    for (int m = 0; m < kPER; m++) {
+        const int idx = tid + kLoadThreads * m;
+        const int r1 = idx / kN2, r2 = idx - r1 * kN2;
+        const int i1 = M.rs1 + r1, i2 = M.rs2 + r2;
+        const bool valid = idx < kPL && i1 >= 0 && i1 < (int)n1 && i2 >= 0 && i2 < (int)n2;
+        const int o1 = valid ? (r1 & 1) : 0, o2 = valid ? (r2 & 1) : 0;  // region starts are even
+        const bool own1 = !act1 || (i1 >= 2 * M.js1 - 1 && i1 < 2 * (M.js1 + M.tj1) - 1);
+        const bool own2 = !act2 || (i2 >= 2 * M.js2 - 1 && i2 < 2 * (M.js2 + M.tj2) - 1);
+        bits |= (uint32_t)(valid | (o1 << 1) | (o2 << 2) | ((own1 && own2) << 3)) << (4 * m);
+        qb[m] = valid ? (o1 ? r1 - 1 : r1) * kN2 + (o2 ? r2 - 1 : r2) : 0;
+        const uint32_t u1 = valid ? (uint32_t)i1 : 0, u2 = valid ? (uint32_t)i2 : 0;
+        in_off[m] = (uint32_t)(u1 * (uint32_t)P.is[1] + u2) * (uint32_t)sizeof(T);
+        const uint32_t p1 = rpos(u1, c1), p2 = rpos(u2, c2);
+        uint32_t a, bb;
+        if (ILV) {
+            if (!o1 && !o2) a = (u1 >> 1) * c2 + (u2 >> 1);  // coarse in even planes
+            else if (p1 < c1) a = p1 * (n2 - c2) + (p2 - c2);
+            else a = c1 * (n2 - c2) + (p1 - c1) * n2 + p2;
+            bb = p1 * n2 + p2;
+        } else {
+            a = (!o1 && !o2) ? (u1 >> 1) * c2 + (u2 >> 1) : p1 * (uint32_t)P.cs[1] + p2;
+            bb = p1 * (uint32_t)P.cs[1] + p2;
+        }
+        offA[m] = a * (uint32_t)sizeof(T);
+        offB[m] = bb * (uint32_t)sizeof(T);
+    }
+    // scalar coefficient base of fine plane i0 (even planes come first in a level)
+    auto coef_base = [&](int i0) __attribute__((always_inline)) -> uint64_t {
+        if (ILV) {
+            if (!act0 || !(i0 & 1)) return (uint64_t)(i0 >> 1) * plane_in;
+            return (uint64_t)c0 * plane_in + (uint64_t)(i0 >> 1) * n12;
+        }
+        return (uint64_t)rpos((uint32_t)i0, c0) * P.cs[0];
     };
-    // thread's coarse column for the accumulation
-    const int cj1 = tid / kTJ2, cj2 = tid % kTJ2;
-    const bool has_col = cj1 < D1.tj && cj2 < D2.tj;
-    double acc[3] = {0.0, 0.0, 0.0};
-    T lmax = 0;
 
-    const int i0lo = (P.n[0] == 1) ? 0 : max(0, 2 * D0.js - 2);
-    const int i0hi = (P.n[0] == 1) ? 0 : min((int)P.n[0] - 1, 2 * (D0.js + D0.tj - 1) + 2);
-
-    auto load_plane = [&](T (*dst)[kR2], int i0) {
-        for (int idx = tid; idx < RR; idx += kLoadThreads) {
-            const int r1 = idx / R2, r2 = idx % R2;
-            const int i1 = D1.rs + r1, i2 = D2.rs + r2;
+    auto fetch = [&](T (&reg)[kPER], int i0) __attribute__((always_inline)) {
+        const bool ok = i0 >= 0 && i0 < (int)n0;
+        const bool odd0 = act0 && (i0 & 1);
+        const char *pb = MODE == kRecompose
+                             ? reinterpret_cast<const char *>((ILV ? P.coef_ilv : P.coef_full) + coef_base(i0))
+                             : reinterpret_cast<const char *>(P.in + (uint64_t)i0 * P.is[0]);
+#pragma unroll
+        for (int m = 0; m < kPER; m++) {
+            const uint32_t b = bits >> (4 * m);
             T v = (T)0;
-            if (i1 >= 0 && i1 < (int)P.n[1] && i2 >= 0 && i2 < (int)P.n[2]) {
+            if (ok && (b & 1)) {
                 if (MODE == kRecompose) {
-                    const bool coarse = !(i0 & 1) && !(i1 & 1) && !(i2 & 1);
-                    if (!coarse) {
-                        if (ILV) v = P.coef_ilv[coef_index_ilv(P, i0, i1, i2)];
-                        else v = P.coef_full[coef_index_full(P, i0, i1, i2)];
-                    }
+                    if (odd0 || (b & 6)) v = *reinterpret_cast<const T *>(pb + (odd0 ? offB[m] : offA[m]));
                 } else {
-                    v = P.in[(uint64_t)i0 * P.is[0] + (uint64_t)i1 * P.is[1] + (uint64_t)i2];
+                    v = *reinterpret_cast<const T *>(pb + in_off[m]);
                 }
             }
-            dst[r1][r2] = v;
+            reg[m] = v;
+        }
+    };
+    auto put = [&](T *dst, const T (&reg)[kPER]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int m = 0; m < kPER; m++) {
+            const int idx = tid + kLoadThreads * m;
+            if (kLoadThreads * (m + 1) <= kPL || idx < kPL) dst[idx] = reg[m];
         }
     };
 
-    // W of plane i0 into Wb (decompose), from planes A (i0-1), B (i0+1), C (i0)
-    auto make_w = [&](int i0, const T (*A)[kR2], const T (*Bp)[kR2], const T (*Cp)[kR2]) {
-        const int o0 = (P.n[0] == 1) ? 0 : (i0 & 1);
-        for (int idx = tid; idx < RR; idx += kLoadThreads) {
-            const int r1 = idx / R2, r2 = idx % R2;
-            const int i1 = D1.rs + r1, i2 = D2.rs + r2;
-            const bool inb = i1 >= 0 && i1 < (int)P.n[1] && i2 >= 0 && i2 < (int)P.n[2];
-            const bool coarse = !o0 && !(r1 & 1) && !(r2 & 1);
-            T w = (T)0;
-            if (inb && !coarse) w = Cp[r1][r2] - interp_lds<T>(A, Bp, Cp, o0, r1, r2);
-            Wb[r1][r2] = w;
-            if (inb && own(D0, i0, P.n[0]) && own(D1, i1, P.n[1]) && own(D2, i2, P.n[2])) {
+    const int cj1 = tid / kM2, cj2 = tid % kM2;  // this thread's coarse column
+    const bool has_col = cj1 < M.tj1 && cj2 < M.tj2;
+    const int j1 = M.js1 + cj1, j2 = M.js2 + cj2;
+    double ra = 0.0, rb = 0.0, rc = 0.0;  // running dim-0 sums (see column)
+    T lmax = 0;
+
+    // load-stencil weights along one dim: restriction of the mass matrix,
+    // 3/h-scaled, [1/4, 3/2, 5/2, 3/2, 1/4] (centre 5/4 at the two ends);
+    // the adjoint uses the interpolation transpose [0, 1/2, 1, 1/2, 0].
+    // Taps outside the grid read zeros from the staged planes.
+    constexpr double kW2 = MODE == kAdjointPT ? 0.0 : 0.25;
+    constexpr double kW1 = MODE == kAdjointPT ? 0.5 : 1.5;
+    auto centre = [](int j, uint32_t c) __attribute__((always_inline)) -> double {
+        if (MODE == kAdjointPT) return 1.0;
+        return (j == 0 || j == (int)c - 1) ? 1.25 : 2.5;
+    };
+    const double w0_2 = centre(M.js2 + cj2, c2), w0_1 = centre(j1, c1);
+
+    // W of plane i0 into Wb from planes A (i0-1), B (i0+1), C (i0) [decompose]
+    auto make_w = [&](int i0, const T *A, const T *Bp, const T *C) __attribute__((always_inline)) {
+        const bool o0 = act0 && (i0 & 1);
+        const bool own0 = !act0 || (i0 >= 2 * M.js0 - 1 && i0 < 2 * (M.js0 + M.tj0) - 1);
+        const T *PA = o0 ? A : C, *PB = o0 ? Bp : C;
+        char *cp = reinterpret_cast<char *>((ILV ? P.coef_ilv : P.coef_full) + coef_base(i0));
+        char *up = reinterpret_cast<char *>(P.uc + (uint64_t)(i0 >> 1) * c1 * c2);
+#pragma unroll
+        for (int m = 0; m < kPER; m++) {
+            const int idx = tid + kLoadThreads * m;
+            if (kLoadThreads * (m + 1) > kPL && idx >= kPL) continue;
+            const uint32_t b = bits >> (4 * m);
+            const int d1 = (b & 2) ? 2 * kN2 : 0, d2 = (b & 4) ? 2 : 0;
+            const int q = qb[m];
+            // branch-free multilinear interpolant: the 8 corners of the cell,
+            // an even dim repeating the same value; duplicates double exactly,
+            // so this equals the per-case tree sum of the distinct corners
+            const T I = (((PA[q] + PA[q + d2]) + (PA[q + d1] + PA[q + d1 + d2])) +
+                         ((PB[q] + PB[q + d2]) + (PB[q + d1] + PB[q + d1 + d2]))) *
+                        (T)0.125;
+            const T cv = C[idx];
+            const bool coarse = !o0 && !(b & 6);
+            const T w = ((b & 1) && !coarse) ? cv - I : (T)0;
+            Wb[idx] = w;
+            if ((b & 1) && (b & 8) && own0) {
                 if (coarse) {
-                    const uint64_t j = ((uint64_t)(i0 >> 1) * P.c[1] + (i1 >> 1)) * P.c[2] + (i2 >> 1);
-                    P.uc[j] = Cp[r1][r2];
+                    *reinterpret_cast<T *>(up + offA[m]) = cv;
                 } else {
-                    if (ILV) P.coef_ilv[coef_index_ilv(P, i0, i1, i2)] = w;
-                    else P.coef_full[coef_index_full(P, i0, i1, i2)] = w;
+                    *reinterpret_cast<T *>(cp + (o0 ? offB[m] : offA[m])) = w;
                     if (w == w) lmax = fabs(w) > lmax ? fabs(w) : lmax;
                 }
             }
         }
     };
-
-    // in-plane restriction of Wb (dims 2 then 1) and accumulation along dim 0
-    auto restrict_plane = [&](int i0) {
-        __syncthreads();
-        const int tj2 = D2.tj;
-        for (int idx = tid; idx < R1 * tj2; idx += kLoadThreads) {
-            const int r1 = idx / tj2, jj2 = idx % tj2;
-            const int j2 = D2.js + jj2;
-            double a = 0.0;
-            if (P.n[2] == 1) {
-                a += 1.0 * (double)Wb[r1][0];
-            } else {
-                for (int k = -2; k <= 2; k++) {
-                    const int i2 = 2 * j2 + k;
-                    if (i2 < 0 || i2 >= (int)P.n[2]) continue;
-                    const double wt = (MODE == kAdjointPT) ? pweight(j2, k, P.c[2]) : rweight(j2, k, P.c[2]);
-                    if (MODE == kAdjointPT && wt == 0.0) continue;
-                    a += wt * (double)Wb[r1][i2 - D2.rs];
-                }
-            }
-            g2[r1][jj2] = a;
-        }
-        __syncthreads();
-        if (has_col) {
-            const int j1 = D1.js + cj1;
-            double h = 0.0;
-            if (P.n[1] == 1) {
-                h += 1.0 * g2[0][cj2];
-            } else {
-                for (int k = -2; k <= 2; k++) {
-                    const int i1 = 2 * j1 + k;
-                    if (i1 < 0 || i1 >= (int)P.n[1]) continue;
-                    const double wt = (MODE == kAdjointPT) ? pweight(j1, k, P.c[1]) : rweight(j1, k, P.c[1]);
-                    if (MODE == kAdjointPT && wt == 0.0) continue;
-                    h += wt * g2[i1 - D1.rs][cj2];
-                }
-            }
-            // dim 0: plane i0 contributes to coarse j0 with |2 j0 - i0| <= 2
-            if (P.n[0] == 1) {
-                acc[0] += 1.0 * h;
-            } else {
-                for (int j0 = (i0 - 1) >> 1; j0 <= (i0 + 2) >> 1; j0++) {
-                    if (j0 < D0.js || j0 >= D0.js + D0.tj) continue;
-                    const int k = i0 - 2 * j0;
-                    if (k < -2 || k > 2) continue;
-                    const double wt = (MODE == kAdjointPT) ? pweight(j0, k, P.c[0]) : rweight(j0, k, P.c[0]);
-                    if (MODE == kAdjointPT && wt == 0.0) continue;
-                    const int sl = j0 % 3;
+    // load stencil along dim 2 from Wb into G2[r1][jj2]; this thread's jj2
+    // is fixed (tid % kM2), its rows step by kLoadThreads / kM2
+    auto stencil2 = [&]() __attribute__((always_inline)) {
+        if (cj2 >= M.tj2) return;
+        constexpr int kRowStep = kLoadThreads / kM2;
 #pragma unroll
-                    for (int q = 0; q < 3; q++)
-                        if (q == sl) acc[q] += wt * h;
-                }
+        for (int m = 0; m < kG2PER; m++) {
+            const int r1 = cj1 + kRowStep * m;
+            if (kLoadThreads * (m + 1) > kG2 && r1 >= kN1) continue;
+            const T *row = Wb + r1 * kN2;
+            double a;
+            if (!act2) {
+                a = (double)row[0];
+            } else {
+                const T *t = row + 2 * cj2;  // taps i2 = 2 jj - 2 .. 2 jj + 2
+                a = kW2 * (double)t[0];
+                a = fma(kW1, (double)t[1], a);
+                a = fma(w0_2, (double)t[2], a);
+                a = fma(kW1, (double)t[3], a);
+                a = fma(kW2, (double)t[4], a);
             }
-            // coarse plane j0 is complete once fine plane min(2 j0 + 2, n0 - 1) is in
-            const int j2 = D2.js + cj2;
-            auto emit = [&](int j0) {
-                if (j0 < D0.js || j0 >= D0.js + D0.tj || j0 < 0) return;
-                const int sl = j0 % 3;
-                double v = 0.0;
-#pragma unroll
-                for (int q = 0; q < 3; q++)
-                    if (q == sl) { v = acc[q]; acc[q] = 0.0; }
-                P.f[((uint64_t)j0 * P.c[1] + j1) * P.c[2] + j2] = v;
-            };
-            if (P.n[0] == 1) {
-                emit(0);
-            } else if (!(i0 & 1)) {
-                if (i0 >= 2) emit((i0 - 2) / 2);
-                if (i0 == (int)P.n[0] - 1) emit(i0 / 2);
-            }
+            G2[r1 * kM2 + cj2] = a;
         }
-        __syncthreads();
+    };
+    // dim 1 for this thread's column (rows 2 cj1 .. 2 cj1 + 4 of G2), then
+    // the running dim-0 sums. Fine plane i0 contributes to coarse planes j0
+    // with |2 j0 - i0| <= 2, in increasing i0. Registers: ra = coarse plane
+    // m-1, rb = m, rc = m+1 for the current even plane 2m; after it, m-1 is
+    // complete and the ring rotates.
+    auto dim1 = [&]() __attribute__((always_inline)) -> double {
+        if (!act1) return G2[cj2];
+        const double *g = G2 + 2 * cj1 * kM2 + cj2;
+        double h = kW2 * g[0];
+        h = fma(kW1, g[kM2], h);
+        h = fma(w0_1, g[2 * kM2], h);
+        h = fma(kW1, g[3 * kM2], h);
+        h = fma(kW2, g[4 * kM2], h);
+        return h;
+    };
+    auto wt0 = [&](int j, int k) __attribute__((always_inline)) -> double {
+        if (k == 0) return centre(j, c0);
+        if (k == 1 || k == -1) return kW1;
+        return kW2;
+    };
+    auto in_tile0 = [&](int j0) __attribute__((always_inline)) -> bool { return j0 >= M.js0 && j0 < M.js0 + M.tj0; };
+    auto store_f = [&](int j0, double v) __attribute__((always_inline)) {
+        P.f[((uint64_t)j0 * c1 + j1) * c2 + j2] = v;
+    };
+    auto column = [&](int i0) __attribute__((always_inline)) {
+        if (!has_col) return;
+        const double h = dim1();
+        if (!act0) {
+            store_f(0, h);
+            return;
+        }
+        const int m = i0 >> 1;
+        if (i0 & 1) {
+            // odd plane 2m+1 (after the rotation: ra = m, rb = m+1)
+            if (in_tile0(m)) ra = fma(wt0(m, 1), h, ra);
+            if (in_tile0(m + 1)) rb = fma(wt0(m + 1, -1), h, rb);
+        } else {
+            // even plane 2m (ra = m-1, rb = m, rc = m+1)
+            if (m >= 1 && in_tile0(m - 1)) ra = fma(wt0(m - 1, 2), h, ra);
+            if (in_tile0(m)) rb = fma(wt0(m, 0), h, rb);
+            if (2 * m + 2 < (int)n0 + 1 && in_tile0(m + 1)) rc = fma(wt0(m + 1, -2), h, rc);
+            if (m >= 1 && in_tile0(m - 1)) store_f(m - 1, ra);
+            if (i0 == (int)n0 - 1 && in_tile0(m)) store_f(m, rb);
+            ra = rb;
+            rb = rc;
+            rc = 0.0;
+        }
     };
 
-    if (MODE == kDecompose) {
-        if (P.n[0] == 1) {
-            load_plane(Ue0, 0);
+    const int i0lo = act0 ? max(0, 2 * M.js0 - 2) : 0;
+    const int i0hi = act0 ? min((int)n0 - 1, 2 * (M.js0 + M.tj0 - 1) + 2) : 0;
+    T pa[kPER], pb[kPER];
+    if constexpr (kDec) {
+        // planes odd i0 and even i0 + 1 per step; E0 holds plane i0 - 1
+        auto step = [&](int i0, T *E0, T *E1) __attribute__((always_inline)) {
+            put(O, pa);
+            put(E1, pb);
+            if (i0 + 2 <= i0hi) { fetch(pa, i0 + 2); fetch(pb, i0 + 3); }
             __syncthreads();
-            make_w(0, Ue0, Ue0, Ue0);
-            restrict_plane(0);
-        } else {
-            load_plane(Ue0, i0lo);
+            make_w(i0, E0, E1, O);
             __syncthreads();
-            make_w(i0lo, Ue0, Ue0, Ue0);
-            restrict_plane(i0lo);
-            T(*E0)[kR2] = Ue0;
-            T(*E1)[kR2] = Ue1;
-            for (int i0 = i0lo + 1; i0 <= i0hi; i0 += 2) {
-                load_plane(Uo, i0);
-                load_plane(E1, i0 + 1);
-                __syncthreads();
-                make_w(i0, E0, E1, Uo);
-                restrict_plane(i0);
-                make_w(i0 + 1, E1, E1, E1);
-                restrict_plane(i0 + 1);
-                T(*t)[kR2] = E0;
-                E0 = E1;
-                E1 = t;
-            }
+            stencil2();
+            __syncthreads();
+            column(i0);
+            make_w(i0 + 1, E1, E1, E1);
+            __syncthreads();
+            stencil2();
+            __syncthreads();
+            column(i0 + 1);
+        };
+        fetch(pa, i0lo);
+        put(Ea, pa);
+        if (i0lo + 1 <= i0hi) { fetch(pa, i0lo + 1); fetch(pb, i0lo + 2); }
+        __syncthreads();
+        make_w(i0lo, Ea, Ea, Ea);
+        __syncthreads();
+        stencil2();
+        __syncthreads();
+        column(i0lo);
+        T *E0 = Ea, *E1 = Eb;
+#pragma unroll 1
+        for (int i0 = i0lo + 1; i0 <= i0hi; i0 += 2) {
+            step(i0, E0, E1);
+            T *t = E0;
+            E0 = E1;
+            E1 = t;
         }
-        // level max |W|
         if (P.key) {
-            K m = abs_key(lmax);
+            K mk = abs_key(lmax);
             for (int o = 32; o >= 1; o >>= 1) {
-                K q = __shfl_xor(m, o, 64);
-                m = q > m ? q : m;
+                K q = __shfl_xor(mk, o, 64);
+                mk = q > mk ? q : mk;
             }
-            if (lane_id() == 0 && m) atomicMax(P.key, m);
+            if (lane_id() == 0 && mk) atomicMax(P.key, mk);
         }
     } else {
         // recompose: W straight from the coefficients; adjoint: g
+        fetch(pa, i0lo);
         for (int i0 = i0lo; i0 <= i0hi; i0++) {
-            load_plane(Wb, i0);
-            restrict_plane(i0);
+            put(Wb, pa);
+            if (i0 + 1 <= i0hi) fetch(pa, i0 + 1);
+            __syncthreads();
+            stencil2();
+            __syncthreads();
+            column(i0);
         }
     }
 }
@@ -464,40 +577,91 @@ __global__ __launch_bounds__(512) void k_solve_strided(SolveParams<T> S, uint32_
 }
 
 // ---- recompose interpolation: u = I u_c + W ------------------------------------
+// One workgroup per fine tile 8 x 8 x 64; its coarse neighbourhood
+// (5 x 5 x 33) is staged in LDS, W is read from the coefficients.
+constexpr int kI0 = 8, kI1 = 8, kI2 = 64;
+constexpr int kC0 = kI0 / 2 + 1, kC1 = kI1 / 2 + 1, kC2 = kI2 / 2 + 1;
+
 template <class T, bool ILV>
 __global__ __launch_bounds__(256) void k_interp(LoadParams<T> P, const T *__restrict__ uc, T *__restrict__ out,
                                                 uint64_t os0, uint64_t os1) {
-    const uint32_t i2 = blockIdx.x * 256 + threadIdx.x;
-    const uint32_t i1 = blockIdx.y;
-    const uint32_t i0 = blockIdx.z;
-    if (i2 >= P.n[2]) return;
-    const uint32_t o0 = (P.n[0] == 1) ? 0 : (i0 & 1), o1 = (P.n[1] == 1) ? 0 : (i1 & 1), o2 = (P.n[2] == 1) ? 0 : (i2 & 1);
-    const uint32_t j0 = i0 >> 1, j1 = i1 >> 1, j2 = i2 >> 1;
+    __shared__ T C[kC0 * kC1 * kC2];
+    const uint32_t t2n = (P.n[2] + kI2 - 1) / kI2, t1n = (P.n[1] + kI1 - 1) / kI1;
+    const uint32_t t0n = (P.n[0] + kI0 - 1) / kI0;
+    const uint32_t bt = xcd_remap(blockIdx.x, t0n * t1n * t2n);
+    const uint32_t s2 = (bt % t2n) * kI2, s1 = ((bt / t2n) % t1n) * kI1, s0 = (bt / (t2n * t1n)) * kI0;
     const uint32_t *c = P.c;
-    auto UC = [&](uint32_t a, uint32_t b, uint32_t d) -> T { return uc[((uint64_t)a * c[1] + b) * c[2] + d]; };
-    T v;
-    const uint32_t cnt = o0 + o1 + o2;
-    if (cnt == 0) {
-        v = UC(j0, j1, j2);
-    } else {
-        T I;
-        if (cnt == 1) {
-            if (o2) I = (UC(j0, j1, j2) + UC(j0, j1, j2 + 1)) * (T)0.5;
-            else if (o1) I = (UC(j0, j1, j2) + UC(j0, j1 + 1, j2)) * (T)0.5;
-            else I = (UC(j0, j1, j2) + UC(j0 + 1, j1, j2)) * (T)0.5;
-        } else if (cnt == 2) {
-            if (!o0) I = ((UC(j0, j1, j2) + UC(j0, j1, j2 + 1)) + (UC(j0, j1 + 1, j2) + UC(j0, j1 + 1, j2 + 1))) * (T)0.25;
-            else if (!o1) I = ((UC(j0, j1, j2) + UC(j0, j1, j2 + 1)) + (UC(j0 + 1, j1, j2) + UC(j0 + 1, j1, j2 + 1))) * (T)0.25;
-            else I = ((UC(j0, j1, j2) + UC(j0, j1 + 1, j2)) + (UC(j0 + 1, j1, j2) + UC(j0 + 1, j1 + 1, j2))) * (T)0.25;
-        } else {
-            I = (((UC(j0, j1, j2) + UC(j0, j1, j2 + 1)) + (UC(j0, j1 + 1, j2) + UC(j0, j1 + 1, j2 + 1))) +
-                 ((UC(j0 + 1, j1, j2) + UC(j0 + 1, j1, j2 + 1)) + (UC(j0 + 1, j1 + 1, j2) + UC(j0 + 1, j1 + 1, j2 + 1)))) *
-                (T)0.125;
-        }
-        const T w = ILV ? P.coef_ilv[coef_index_ilv(P, i0, i1, i2)] : P.coef_full[coef_index_full(P, i0, i1, i2)];
-        v = (T)(w + I);
+    // coarse region [s/2, s/2 + C) clipped
+    const uint32_t c0s = s0 >> 1, c1s = s1 >> 1, c2s = s2 >> 1;
+    const uint32_t e0 = min(c[0] - c0s, (uint32_t)kC0), e1 = min(c[1] - c1s, (uint32_t)kC1),
+                   e2 = min(c[2] - c2s, (uint32_t)kC2);
+    for (uint32_t idx = threadIdx.x; idx < e0 * e1 * e2; idx += 256) {
+        const uint32_t a2 = idx % e2, a1 = (idx / e2) % e1, a0 = idx / (e2 * e1);
+        C[(a0 * kC1 + a1) * kC2 + a2] = uc[((uint64_t)(c0s + a0) * c[1] + (c1s + a1)) * c[2] + (c2s + a2)];
     }
-    out[(uint64_t)i0 * os0 + (uint64_t)i1 * os1 + i2] = v;
+    __syncthreads();
+    // a wave is one row of 64 along dim 2: row quantities are wave-uniform
+    // (scalar), the lane supplies i2
+    const uint32_t lane = threadIdx.x % kI2;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kI2);
+    const uint32_t i2 = s2 + lane;
+    const bool in2 = i2 < P.n[2];
+    const uint32_t o2 = (P.n[2] == 1) ? 0 : (i2 & 1);
+    const uint32_t p2 = rpos(i2, c[2]);
+    const uint32_t n1 = P.n[1], n2 = P.n[2];
+    const uint64_t n12 = (uint64_t)n1 * n2, plane_in = n12 - (uint64_t)c[1] * c[2];
+    const uint32_t la2 = (i2 >> 1) - c2s, lb2 = la2 + o2;  // LDS columns of the two dim-2 corners
+    constexpr int kWaves = 256 / kI2;
+    constexpr int kRows = kI0 * kI1 / kWaves;  // rows per wave
+    // issue every coefficient load of this thread before computing
+    T wv[kRows];
+#pragma unroll
+    for (int q = 0; q < kRows; q++) {
+        const uint32_t row = wave + q * kWaves;
+        const uint32_t i1 = s1 + row % kI1, i0 = s0 + row / kI1;
+        wv[q] = (T)0;
+        if (i0 < P.n[0] && i1 < n1 && in2) {
+            const bool e0 = P.n[0] == 1 || !(i0 & 1), e1 = n1 == 1 || !(i1 & 1);
+            const bool coarse = e0 && e1 && !o2;
+            if (!coarse) {
+                if (ILV) {
+                    // DirectInterleaver order: scalar row base + lane offset
+                    const uint32_t p0 = rpos(i0, c[0]), p1 = rpos(i1, c[1]);
+                    uint64_t base;
+                    uint32_t off;
+                    if (p0 < c[0]) {
+                        base = (uint64_t)p0 * plane_in;
+                        if (p1 < c[1]) { base += (uint64_t)p1 * (n2 - c[2]); off = p2 - c[2]; }
+                        else { base += (uint64_t)c[1] * (n2 - c[2]) + (uint64_t)(p1 - c[1]) * n2; off = p2; }
+                    } else {
+                        base = (uint64_t)c[0] * plane_in + (uint64_t)(p0 - c[0]) * n12 + (uint64_t)p1 * n2;
+                        off = p2;
+                    }
+                    wv[q] = P.coef_ilv[base + off];
+                } else {
+                    wv[q] = P.coef_full[coef_index_full(P, i0, i1, i2)];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kRows; q++) {
+        const uint32_t row = wave + q * kWaves;
+        const uint32_t i1 = s1 + row % kI1, i0 = s0 + row / kI1;
+        if (i0 >= P.n[0] || i1 >= n1 || !in2) continue;
+        const uint32_t o0 = (P.n[0] == 1) ? 0 : (i0 & 1), o1 = (n1 == 1) ? 0 : (i1 & 1);
+        // 8 corners with repeats along even dims (exact doubling): equals the
+        // per-case tree sum of the distinct coarse corners
+        const uint32_t ra0 = ((i0 >> 1) - c0s) * kC1, rb0 = ra0 + o0 * kC1;
+        const uint32_t ra1 = (i1 >> 1) - c1s, rb1 = ra1 + o1;
+        const T *Caa = C + (ra0 + ra1) * kC2, *Cab = C + (ra0 + rb1) * kC2;
+        const T *Cba = C + (rb0 + ra1) * kC2, *Cbb = C + (rb0 + rb1) * kC2;
+        const T I = (((Caa[la2] + Caa[lb2]) + (Cab[la2] + Cab[lb2])) +
+                     ((Cba[la2] + Cba[lb2]) + (Cbb[la2] + Cbb[lb2]))) *
+                    (T)0.125;
+        const T v = (o0 + o1 + o2 == 0) ? Caa[la2] : (T)(wv[q] + I);
+        out[(uint64_t)i0 * os0 + (uint64_t)i1 * os1 + i2] = v;
+    }
 }
 
 // ---- box copies -----------------------------------------------------------------
@@ -626,10 +790,15 @@ LoadParams<T> load_params(const LevelGeom &g, int l, const CoefSink<T> &sink) {
 }
 
 template <class T, int MODE>
-int launch_load(const LoadParams<T> &P, bool ilv, hipStream_t s) {
-    const uint32_t t2n = (P.n[2] == 1) ? 1 : div_up(P.c[2], kTJ2);
-    const uint32_t t1n = (P.n[1] == 1) ? 1 : div_up(P.c[1], kTJ1);
-    const uint32_t t0n = (P.n[0] == 1) ? 1 : div_up(P.c[0], kTJ0);
+int launch_load(LoadParams<T> P, bool ilv, hipStream_t s) {
+    const uint32_t t2n = (P.n[2] == 1) ? 1 : div_up(P.c[2], kM2);
+    const uint32_t t1n = (P.n[1] == 1) ? 1 : div_up(P.c[1], kM1);
+    // march length: 16 coarse planes per workgroup, shorter when the level
+    // has too few tiles to fill the 256 CUs (short marches = more parallelism)
+    int march = 16;
+    while (march > 2 && (uint64_t)t2n * t1n * div_up(P.c[0], march) < 2048) march /= 2;
+    P.march = march;
+    const uint32_t t0n = (P.n[0] == 1) ? 1 : div_up(P.c[0], march);
     const uint32_t grid = t2n * t1n * t0n;
     if (ilv) hipLaunchKernelGGL((k_load<T, MODE, true>), dim3(grid), dim3(kLoadThreads), 0, s, P);
     else hipLaunchKernelGGL((k_load<T, MODE, false>), dim3(grid), dim3(kLoadThreads), 0, s, P);
@@ -641,9 +810,18 @@ int launch_load(const LoadParams<T> &P, bool ilv, hipStream_t s) {
 
 size_t mgard_workspace_bytes(const LevelGeom &g, size_t elem) { return ws_layout(g, elem).total; }
 
+int mgard_upload_tables(const LevelGeom &g, size_t elem, void *ws, size_t ws_bytes) {
+    const WsLayout w = ws_layout(g, elem);
+    if (ws_bytes < w.total) { set_error("workspace too small"); return kErrArg; }
+    const Factors F = make_factors(g);
+    if (!F.host.empty())
+        PRODM_HIP(hipMemcpy(static_cast<char *>(ws) + w.tab, F.host.data(), F.host.size() * 8, hipMemcpyHostToDevice));
+    return kOk;
+}
+
 template <class T>
 int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g, const CoefSink<T> &sink,
-                    KeyOf<T> *level_keys, void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm) {
+                    KeyOf<T> *level_keys, void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm, bool tables_ready) {
     const WsLayout w = ws_layout(g, sizeof(T));
     if (ws_bytes < w.total) { set_error("decompose: workspace too small"); return kErrArg; }
     char *base = static_cast<char *>(ws);
@@ -651,7 +829,10 @@ int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g,
     double *f = reinterpret_cast<double *>(base + w.f);
     double *tab = reinterpret_cast<double *>(base + w.tab);
     const Factors F = make_factors(g);
-    if (!F.host.empty()) PRODM_HIP(hipMemcpyAsync(tab, F.host.data(), F.host.size() * 8, hipMemcpyHostToDevice, s));
+    // a copy from pageable memory blocks the host until the stream drains:
+    // plans upload the tables once (mgard_upload_tables)
+    if (!tables_ready && !F.host.empty())
+        PRODM_HIP(hipMemcpyAsync(tab, F.host.data(), F.host.size() * 8, hipMemcpyHostToDevice, s));
     const bool ilv = sink.full == nullptr;
     const T *cur = in;
     uint64_t cs[3] = {in_strides[0], in_strides[1], 1};
@@ -699,12 +880,12 @@ int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g,
     return kOk;
 }
 template int mgard_decompose<float>(const float *, const uint64_t *, const LevelGeom &, const CoefSink<float> &,
-                                    uint32_t *, void *, size_t, hipStream_t, StageTimer *);
+                                    uint32_t *, void *, size_t, hipStream_t, StageTimer *, bool);
 template int mgard_decompose<double>(const double *, const uint64_t *, const LevelGeom &, const CoefSink<double> &,
-                                     unsigned long long *, void *, size_t, hipStream_t, StageTimer *);
+                                     unsigned long long *, void *, size_t, hipStream_t, StageTimer *, bool);
 
 template <class T>
-int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int /*unused*/, T *out, const uint64_t *out_strides,
+int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int tables_ready, T *out, const uint64_t *out_strides,
                     void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm) {
     const WsLayout w = ws_layout(g, sizeof(T));
     if (ws_bytes < w.total) { set_error("recompose: workspace too small"); return kErrArg; }
@@ -713,7 +894,8 @@ int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int /*unused*/, 
     double *f = reinterpret_cast<double *>(base + w.f);
     double *tab = reinterpret_cast<double *>(base + w.tab);
     const Factors F = make_factors(g);
-    if (!F.host.empty()) PRODM_HIP(hipMemcpyAsync(tab, F.host.data(), F.host.size() * 8, hipMemcpyHostToDevice, s));
+    if (!tables_ready && !F.host.empty())
+        PRODM_HIP(hipMemcpyAsync(tab, F.host.data(), F.host.size() * 8, hipMemcpyHostToDevice, s));
     const bool ilv = src.full == nullptr;
     const uint32_t *c0 = g.ld[0];
     if (g.L == 0) {
@@ -745,11 +927,11 @@ int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int /*unused*/, 
         uint64_t d0, d1;
         if (l == g.L) { dst = out; d0 = out_strides[0]; d1 = out_strides[1]; }
         else { dst = bufs[which ^ 1]; d0 = (uint64_t)n[1] * n[2]; d1 = n[2]; }
-        dim3 grid(div_up(n[2], 256), n[1], n[0]);
+        const uint32_t grid = div_up(n[2], kI2) * div_up(n[1], kI1) * div_up(n[0], kI0);
         {
             StageScope sc(tm, l == g.L ? kStRecInterpTop : kStRecInterpRest);
-            if (ilv) hipLaunchKernelGGL((k_interp<T, true>), grid, dim3(256), 0, s, P, cur, dst, d0, d1);
-            else hipLaunchKernelGGL((k_interp<T, false>), grid, dim3(256), 0, s, P, cur, dst, d0, d1);
+            if (ilv) hipLaunchKernelGGL((k_interp<T, true>), dim3(grid), dim3(256), 0, s, P, cur, dst, d0, d1);
+            else hipLaunchKernelGGL((k_interp<T, false>), dim3(grid), dim3(256), 0, s, P, cur, dst, d0, d1);
         }
         PRODM_CHECK_LAUNCH();
         which ^= 1;
@@ -757,6 +939,35 @@ int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int /*unused*/, 
     }
     return kOk;
 }
+// Adjoint helpers (adjoint.hip): h = P^T g on the coarse grid of level l
+// (the load kernel with the interpolation-transpose weights), copied to
+// h_out, then t = S h in place in the workspace's load vector.
+int mgard_adjoint_coarse(const double *gfine, const LevelGeom &g, int l, double *h_out, void *ws, size_t ws_bytes,
+                         hipStream_t s) {
+    const WsLayout w = ws_layout(g, sizeof(double));
+    if (ws_bytes < w.total) { set_error("adjoint: workspace too small"); return kErrArg; }
+    char *base = static_cast<char *>(ws);
+    double *f = reinterpret_cast<double *>(base + w.f);
+    double *tab = reinterpret_cast<double *>(base + w.tab);
+    const Factors F = make_factors(g);
+    if (l == g.L && !F.host.empty())
+        PRODM_HIP(hipMemcpyAsync(tab, F.host.data(), F.host.size() * 8, hipMemcpyHostToDevice, s));
+    LoadParams<double> P{};
+    for (int d = 0; d < 3; d++) { P.n[d] = g.ld[l][d]; P.c[d] = g.ld[l - 1][d]; P.ilv_c[d] = g.ld[l - 1][d]; }
+    P.in = gfine;
+    P.is[0] = (uint64_t)P.n[1] * P.n[2];
+    P.is[1] = P.n[2];
+    P.is[2] = 1;
+    P.f = f;
+    int rc = launch_load<double, kAdjointPT>(P, false, s);
+    if (rc) return rc;
+    PRODM_HIP(hipMemcpyAsync(h_out, f, points3(g.ld[l - 1]) * sizeof(double), hipMemcpyDeviceToDevice, s));
+    return solve_all<double>(g, l, f, tab, F, kEpiNone, nullptr, nullptr, s);
+}
+const double *mgard_adjoint_solution(const LevelGeom &g, void *ws) {
+    return reinterpret_cast<const double *>(static_cast<char *>(ws) + ws_layout(g, sizeof(double)).f);
+}
+
 template int mgard_recompose<float>(const CoefSink<float> &, const LevelGeom &, int, float *, const uint64_t *, void *,
                                     size_t, hipStream_t, StageTimer *);
 template int mgard_recompose<double>(const CoefSink<double> &, const LevelGeom &, int, double *, const uint64_t *,

@@ -73,6 +73,20 @@ def recompose(c, dims, L, strides=None):
     return out
 
 
+def recompose_adjoint(g, dims, L, strides=None):
+    """dJ/dcoefficients (decompose layout) from dJ/du, the transpose of
+    recompose; FP64 inside."""
+    out = torch.empty_like(g)
+    nb = lib().prodm_recompose_adjoint_workspace_bytes(_u32(dims), len(dims), L)
+    if nb == 0:
+        raise capi.ProdmError("unsupported geometry")
+    ws = _ws(nb)
+    st = _u32(strides) if strides is not None else None
+    check(_fn("prodm_recompose_adjoint", g.dtype)(_ptr(g), _ptr(out), _u32(dims), len(dims), L, st, _ptr(ws),
+                                                  ws.numel(), _stream()), "recompose_adjoint")
+    return out
+
+
 # ---- interleave ------------------------------------------------------------------
 def interleave(data, dims, fine, coarse, strides=None):
     n = int(np.prod(fine, dtype=np.int64) - np.prod(coarse, dtype=np.int64))
