@@ -8,7 +8,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libprodm_mdr.so")
+# PRODM_MDR_LIB selects another build of the same library (same ABI)
+LIB_PATH = os.environ.get("PRODM_MDR_LIB") or os.path.join(HERE, "libprodm_mdr.so")
 
 F32, F64 = 0, 1
 NEGABINARY, GROUPED = 0, 1

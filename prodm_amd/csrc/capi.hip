@@ -4,7 +4,7 @@
 #include <vector>
 
 #include "internal.hpp"
-#include "../../include/prodm_mdr.h"
+#include "prodm_mdr.h"
 
 namespace prodm {
 

@@ -22,6 +22,7 @@
 #include "internal.hpp"
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 namespace prodm {
@@ -268,14 +269,15 @@ __global__ __launch_bounds__(kLoadThreads) void k_load(LoadParams<T> P) {
             const bool coarse = !o0 && !(b & 6);
             const T w = ((b & 1) && !coarse) ? cv - I : (T)0;
             Wb[idx] = w;
-            if ((b & 1) && (b & 8) && own0) {
-                if (coarse) {
-                    *reinterpret_cast<T *>(up + offA[m]) = cv;
-                } else {
-                    *reinterpret_cast<T *>(cp + (o0 ? offB[m] : offA[m])) = w;
-                    if (w == w) lmax = fabs(w) > lmax ? fabs(w) : lmax;
-                }
-            }
+            // one predicated store: the coarse value into u_c or the
+            // coefficient into the level buffer
+            const bool st = (b & 1) && (b & 8) && own0;
+            const uint32_t oa = offA[m], ob = offB[m];
+            const uint32_t oc = o0 ? ob : oa;
+            char *dst = coarse ? up + oa : cp + oc;
+            if (st) *reinterpret_cast<T *>(dst) = coarse ? cv : w;
+            const T aw = (st && !coarse && w == w) ? fabs(w) : (T)0;
+            lmax = aw > lmax ? aw : lmax;
         }
     };
     // load stencil along dim 2 from Wb into G2[r1][jj2]; this thread's jj2
@@ -600,68 +602,74 @@ __global__ __launch_bounds__(256) void k_interp(LoadParams<T> P, const T *__rest
         C[(a0 * kC1 + a1) * kC2 + a2] = uc[((uint64_t)(c0s + a0) * c[1] + (c1s + a1)) * c[2] + (c2s + a2)];
     }
     __syncthreads();
-    // a wave is one row of 64 along dim 2: row quantities are wave-uniform
-    // (scalar), the lane supplies i2
-    const uint32_t lane = threadIdx.x % kI2;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kI2);
+    // a wave handles rows of 64 along dim 2; every address is a tile base
+    // pointer plus a 32-bit byte offset (row part + lane part)
+    const uint32_t lane = threadIdx.x % kI2, wave = threadIdx.x / kI2;
+    const uint32_t n0 = P.n[0], n1 = P.n[1], n2 = P.n[2];
     const uint32_t i2 = s2 + lane;
-    const bool in2 = i2 < P.n[2];
-    const uint32_t o2 = (P.n[2] == 1) ? 0 : (i2 & 1);
+    const bool in2 = i2 < n2;
+    const uint32_t o2 = (n2 == 1) ? 0 : (i2 & 1);
     const uint32_t p2 = rpos(i2, c[2]);
-    const uint32_t n1 = P.n[1], n2 = P.n[2];
-    const uint64_t n12 = (uint64_t)n1 * n2, plane_in = n12 - (uint64_t)c[1] * c[2];
+    const uint32_t n12 = n1 * n2, plane_in = n12 - c[1] * c[2];
+    const uint32_t h0 = s0 >> 1;  // (i0 >> 1) of the tile's first plane
+    const char *ce = nullptr, *co = nullptr;
+    if (ILV) {
+        ce = reinterpret_cast<const char *>(P.coef_ilv + (uint64_t)h0 * plane_in);  // even planes (p0 < c0)
+        co = reinterpret_cast<const char *>(P.coef_ilv + (uint64_t)c[0] * plane_in + (uint64_t)h0 * n12);  // odd
+    }
+    char *ot = reinterpret_cast<char *>(out + (uint64_t)s0 * os0 + (uint64_t)s1 * os1 + s2);
     const uint32_t la2 = (i2 >> 1) - c2s, lb2 = la2 + o2;  // LDS columns of the two dim-2 corners
     constexpr int kWaves = 256 / kI2;
     constexpr int kRows = kI0 * kI1 / kWaves;  // rows per wave
-    // issue every coefficient load of this thread before computing
-    T wv[kRows];
+    // tiles inside the grid (all but the last along each dim) skip every
+    // bounds test; coarse lanes load a harmless in-buffer word and discard it
+    const bool full = s0 + kI0 <= n0 && s1 + kI1 <= n1 && s2 + kI2 <= n2;
+    auto body = [&](auto full_t) __attribute__((always_inline)) {
+        constexpr bool FULL = decltype(full_t)::value;
+        T wv[kRows];
 #pragma unroll
-    for (int q = 0; q < kRows; q++) {
-        const uint32_t row = wave + q * kWaves;
-        const uint32_t i1 = s1 + row % kI1, i0 = s0 + row / kI1;
-        wv[q] = (T)0;
-        if (i0 < P.n[0] && i1 < n1 && in2) {
-            const bool e0 = P.n[0] == 1 || !(i0 & 1), e1 = n1 == 1 || !(i1 & 1);
-            const bool coarse = e0 && e1 && !o2;
-            if (!coarse) {
-                if (ILV) {
-                    // DirectInterleaver order: scalar row base + lane offset
-                    const uint32_t p0 = rpos(i0, c[0]), p1 = rpos(i1, c[1]);
-                    uint64_t base;
-                    uint32_t off;
-                    if (p0 < c[0]) {
-                        base = (uint64_t)p0 * plane_in;
-                        if (p1 < c[1]) { base += (uint64_t)p1 * (n2 - c[2]); off = p2 - c[2]; }
-                        else { base += (uint64_t)c[1] * (n2 - c[2]) + (uint64_t)(p1 - c[1]) * n2; off = p2; }
-                    } else {
-                        base = (uint64_t)c[0] * plane_in + (uint64_t)(p0 - c[0]) * n12 + (uint64_t)p1 * n2;
-                        off = p2;
-                    }
-                    wv[q] = P.coef_ilv[base + off];
-                } else {
-                    wv[q] = P.coef_full[coef_index_full(P, i0, i1, i2)];
-                }
+        for (int q = 0; q < kRows; q++) {
+            const uint32_t row = wave + q * kWaves;
+            const uint32_t i1 = s1 + row % kI1, i0 = s0 + row / kI1;
+            const bool odd0 = n0 > 1 && (i0 & 1), e1 = n1 == 1 || !(i1 & 1);
+            const bool coarse = !odd0 && e1 && !o2;
+            const bool ok = (FULL || (i0 < n0 && i1 < n1 && in2)) && !coarse;
+            T v;
+            if (ILV) {
+                // DirectInterleaver order inside the level buffer
+                const uint32_t k0 = (i0 >> 1) - h0, p1 = rpos(i1, c[1]);
+                const uint32_t off = !odd0 ? k0 * plane_in + (p1 < c[1] ? p1 * (n2 - c[2]) + (p2 - c[2])
+                                                                        : c[1] * (n2 - c[2]) + (p1 - c[1]) * n2 + p2)
+                                           : k0 * n12 + p1 * n2 + p2;
+                const char *src = (ok && odd0) ? co : ce;
+                v = *reinterpret_cast<const T *>(src + (ok ? off * (uint32_t)sizeof(T) : 0u));
+            } else {
+                v = P.coef_full[ok ? coef_index_full(P, i0, i1, i2) : 0];
             }
+            wv[q] = ok ? v : (T)0;
         }
-    }
 #pragma unroll
-    for (int q = 0; q < kRows; q++) {
-        const uint32_t row = wave + q * kWaves;
-        const uint32_t i1 = s1 + row % kI1, i0 = s0 + row / kI1;
-        if (i0 >= P.n[0] || i1 >= n1 || !in2) continue;
-        const uint32_t o0 = (P.n[0] == 1) ? 0 : (i0 & 1), o1 = (n1 == 1) ? 0 : (i1 & 1);
-        // 8 corners with repeats along even dims (exact doubling): equals the
-        // per-case tree sum of the distinct coarse corners
-        const uint32_t ra0 = ((i0 >> 1) - c0s) * kC1, rb0 = ra0 + o0 * kC1;
-        const uint32_t ra1 = (i1 >> 1) - c1s, rb1 = ra1 + o1;
-        const T *Caa = C + (ra0 + ra1) * kC2, *Cab = C + (ra0 + rb1) * kC2;
-        const T *Cba = C + (rb0 + ra1) * kC2, *Cbb = C + (rb0 + rb1) * kC2;
-        const T I = (((Caa[la2] + Caa[lb2]) + (Cab[la2] + Cab[lb2])) +
-                     ((Cba[la2] + Cba[lb2]) + (Cbb[la2] + Cbb[lb2]))) *
-                    (T)0.125;
-        const T v = (o0 + o1 + o2 == 0) ? Caa[la2] : (T)(wv[q] + I);
-        out[(uint64_t)i0 * os0 + (uint64_t)i1 * os1 + i2] = v;
-    }
+        for (int q = 0; q < kRows; q++) {
+            const uint32_t row = wave + q * kWaves;
+            const uint32_t r1 = row % kI1, r0 = row / kI1;
+            const uint32_t i1 = s1 + r1, i0 = s0 + r0;
+            if (!FULL && (i0 >= n0 || i1 >= n1 || !in2)) continue;
+            const uint32_t o0 = (n0 == 1) ? 0 : (i0 & 1), o1 = (n1 == 1) ? 0 : (i1 & 1);
+            // 8 corners with repeats along even dims (exact doubling): equals
+            // the per-case tree sum of the distinct coarse corners
+            const uint32_t ra0 = ((i0 >> 1) - c0s) * kC1, rb0 = ra0 + o0 * kC1;
+            const uint32_t ra1 = (i1 >> 1) - c1s, rb1 = ra1 + o1;
+            const T *Caa = C + (ra0 + ra1) * kC2, *Cab = C + (ra0 + rb1) * kC2;
+            const T *Cba = C + (rb0 + ra1) * kC2, *Cbb = C + (rb0 + rb1) * kC2;
+            const T I = (((Caa[la2] + Caa[lb2]) + (Cab[la2] + Cab[lb2])) +
+                         ((Cba[la2] + Cba[lb2]) + (Cbb[la2] + Cbb[lb2]))) *
+                        (T)0.125;
+            const T v = (o0 + o1 + o2 == 0) ? Caa[la2] : (T)(wv[q] + I);
+            *reinterpret_cast<T *>(ot + (r0 * (uint32_t)os0 + r1 * (uint32_t)os1 + lane) * (uint32_t)sizeof(T)) = v;
+        }
+    };
+    if (full) body(std::true_type{});
+    else body(std::false_type{});
 }
 
 // ---- box copies -----------------------------------------------------------------
