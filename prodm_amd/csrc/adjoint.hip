@@ -34,16 +34,17 @@ __global__ __launch_bounds__(256) void k_to_double_f64(const double *__restrict_
     out[((uint64_t)blockIdx.z * n1 + blockIdx.y) * n2 + i2] = in[(uint64_t)blockIdx.z * s0 + (uint64_t)blockIdx.y * s1 + i2];
 }
 
-// v = P t: multilinear prolongation of the dense coarse grid t (c) to the
-// dense fine grid v (n), same corner formula as the recompose interpolation
-__global__ __launch_bounds__(256) void k_prolong(const double *__restrict__ t, uint32_t c1, uint32_t c2,
+// v = P t: multilinear prolongation of the coarse grid t (c, rows of pitch
+// tp) to the dense fine grid v (n), same corner formula as the recompose
+// interpolation
+__global__ __launch_bounds__(256) void k_prolong(const double *__restrict__ t, uint32_t c1, uint32_t tp,
                                                   double *__restrict__ v, uint32_t n0, uint32_t n1, uint32_t n2) {
     const uint32_t i2 = blockIdx.x * 256 + threadIdx.x;
     if (i2 >= n2) return;
     const uint32_t i1 = blockIdx.y, i0 = blockIdx.z;
     const uint32_t o0 = n0 == 1 ? 0 : (i0 & 1), o1 = n1 == 1 ? 0 : (i1 & 1), o2 = n2 == 1 ? 0 : (i2 & 1);
     const uint32_t a0 = i0 >> 1, b0 = a0 + o0, a1 = i1 >> 1, b1 = a1 + o1, a2 = i2 >> 1, b2 = a2 + o2;
-    auto T = [&](uint32_t j0, uint32_t j1, uint32_t j2) { return t[((uint64_t)j0 * c1 + j1) * c2 + j2]; };
+    auto T = [&](uint32_t j0, uint32_t j1, uint32_t j2) { return t[((uint64_t)j0 * c1 + j1) * tp + j2]; };
     const double I = (((T(a0, a1, a2) + T(a0, a1, b2)) + (T(a0, b1, a2) + T(a0, b1, b2))) +
                       ((T(b0, a1, a2) + T(b0, a1, b2)) + (T(b0, b1, a2) + T(b0, b1, b2)))) *
                      0.125;
@@ -139,9 +140,10 @@ int mgard_recompose_adjoint(const T *g_in, const uint64_t *in_strides, const Lev
         double *h = hb[which];
         // h = P^T g and t = S h (the transposed load and the coarse solves)
         if (int rc = mgard_adjoint_coarse(cur, g, l, h, mws, mws_bytes, s)) return rc;
-        const double *t = mgard_adjoint_solution(g, mws);
+        uint32_t tp = 0;
+        const double *t = mgard_adjoint_solution(g, l, mws, &tp);
         // r = M P t, then dJ/dW = g - r at the coefficient nodes
-        hipLaunchKernelGGL(k_prolong, grid_of(n), dim3(256), 0, s, t, c[1], c[2], v1, n[0], n[1], n[2]);
+        hipLaunchKernelGGL(k_prolong, grid_of(n), dim3(256), 0, s, t, c[1], tp, v1, n[0], n[1], n[2]);
         hipLaunchKernelGGL(k_mass, grid_of(n), dim3(256), 0, s, v1, v2, n[0], n[1], n[2], 2);
         hipLaunchKernelGGL(k_mass, grid_of(n), dim3(256), 0, s, v2, v1, n[0], n[1], n[2], 1);
         hipLaunchKernelGGL(k_mass, grid_of(n), dim3(256), 0, s, v1, v2, n[0], n[1], n[2], 0);

@@ -114,6 +114,7 @@ int mgard_recompose_adjoint(const T *g_in, const uint64_t *in_strides, const Lev
                             const uint64_t *out_strides, void *ws, size_t ws_bytes, hipStream_t s);
 int mgard_adjoint_coarse(const double *gfine, const LevelGeom &g, int l, double *h_out, void *ws, size_t ws_bytes,
                          hipStream_t s);
-const double *mgard_adjoint_solution(const LevelGeom &g, void *ws);
+// t = S h of level l (rows of `pitch` doubles) in the workspace
+const double *mgard_adjoint_solution(const LevelGeom &g, int l, void *ws, uint32_t *pitch);
 
 }  // namespace prodm

@@ -45,6 +45,10 @@ __host__ __device__ inline double pweight(int j, int k, uint32_t c) {
     return 0.0;
 }
 
+// Row pitch (elements) of the FP64 load vector on a coarse grid: rows start
+// on 128-byte boundaries, so the line solves' 64-lane accesses are aligned.
+__host__ __device__ inline uint32_t f_pitch(uint32_t c2) { return (c2 + 15u) & ~15u; }
+
 template <class T> struct LoadParams {
     uint32_t n[3], c[3];
     const T *in;           // natural-order input box (decompose / adjoint)
@@ -54,7 +58,8 @@ template <class T> struct LoadParams {
     T *coef_ilv;           // interleaved level buffer
     uint32_t ilv_c[3];     // coarse dims used by the interleave index
     T *uc;                 // decompose: coarse nodal values (dense c)
-    double *f;             // load vector (dense c)
+    double *f;             // load vector (c0 x c1 rows of pitch fp)
+    uint32_t fp;           // row pitch of f
     KeyOf<T> *key;         // decompose: level max |W|
     int march;             // coarse planes per workgroup along dim 0
 };
@@ -326,7 +331,7 @@ __global__ __launch_bounds__(kLoadThreads) void k_load(LoadParams<T> P) {
     };
     auto in_tile0 = [&](int j0) __attribute__((always_inline)) -> bool { return j0 >= M.js0 && j0 < M.js0 + M.tj0; };
     auto store_f = [&](int j0, double v) __attribute__((always_inline)) {
-        P.f[((uint64_t)j0 * c1 + j1) * c2 + j2] = v;
+        P.f[((uint64_t)j0 * c1 + j1) * P.fp + j2] = v;
     };
     auto column = [&](int i0) __attribute__((always_inline)) {
         if (!has_col) return;
@@ -428,7 +433,8 @@ static void thomas_factors(uint32_t m, double *inv, double *cp) {
 enum Epilogue : int { kEpiNone = 0, kEpiAdd = 1, kEpiSub = 2 };
 
 template <class T> struct SolveParams {
-    double *f;            // dense c0 x c1 x c2
+    double *f;            // c0 x c1 rows of pitch fp
+    uint32_t fp;
     uint32_t c[3];
     const double *inv, *cp;
     int epi;
@@ -436,26 +442,37 @@ template <class T> struct SolveParams {
     T *uc_out;            // epilogue output (dense c)
 };
 
-template <class T> __device__ inline void epilogue(const SolveParams<T> &S, uint64_t idx, double z) {
-    if (S.epi == kEpiAdd) S.uc_out[idx] = (T)((double)S.uc_in[idx] + z);
-    else if (S.epi == kEpiSub) S.uc_out[idx] = (T)((double)S.uc_in[idx] - z);
-    else S.f[idx] = z;
+// fidx: position in f (padded rows), uidx: position in the dense u_c
+template <class T> __device__ inline void epilogue(const SolveParams<T> &S, uint64_t fidx, uint64_t uidx, double z) {
+    if (S.epi == kEpiAdd) S.uc_out[uidx] = (T)((double)S.uc_in[uidx] + z);
+    else if (S.epi == kEpiSub) S.uc_out[uidx] = (T)((double)S.uc_in[uidx] - z);
+    else S.f[fidx] = z;
 }
 
 // contiguous lines (dim 2): one wave per line, lane holds a segment of SEG
 template <class T, int SEG>
 __global__ __launch_bounds__(256) void k_solve_contig(SolveParams<T> S, uint64_t nlines) {
+    // the line moves between memory and LDS with coalesced 64-lane accesses;
+    // each lane then solves its contiguous segment from LDS
+    __shared__ double buf[4][64 * SEG];
     const uint32_t m = S.c[2];
-    const uint32_t lane = lane_id();
-    const uint64_t line = (uint64_t)blockIdx.x * 4 + threadIdx.x / 64;
+    const uint32_t lane = lane_id(), wv = threadIdx.x / 64;
+    const uint64_t line = (uint64_t)blockIdx.x * 4 + wv;
     if (line >= nlines) return;
-    double *x = S.f + line * m;
+    double *L = buf[wv];
+    const double *x = S.f + line * S.fp;
+#pragma unroll
+    for (int k = 0; k < SEG; k++) {
+        const uint32_t j = k * 64 + lane;
+        if (j < m) L[j] = x[j];
+    }
+    __builtin_amdgcn_wave_barrier();
     const uint32_t seg = (m + 63) / 64;
     const uint32_t b = lane * seg;
     const uint32_t e = min(m, b + seg);
     double v[SEG];
 #pragma unroll
-    for (int i = 0; i < SEG; i++) v[i] = (b + i < e) ? x[b + i] : 0.0;
+    for (int i = 0; i < SEG; i++) v[i] = (b + i < e) ? L[b + i] : 0.0;
     // forward: y_i = (f_i - y_{i-1}) * inv_i  == affine map y_{i-1} -> y_i
     double A = 0.0, Bm = 1.0;
 #pragma unroll
@@ -503,23 +520,43 @@ __global__ __launch_bounds__(256) void k_solve_contig(SolveParams<T> S, uint64_t
             v[i] = z;
         }
     }
+    __builtin_amdgcn_wave_barrier();
 #pragma unroll
     for (int i = 0; i < SEG; i++)
-        if (b + i < e) epilogue(S, line * m + b + i, v[i]);
+        if (b + i < e) L[b + i] = v[i];
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int k = 0; k < SEG; k++) {
+        const uint32_t j = k * 64 + lane;
+        if (j < m) epilogue(S, line * S.fp + j, line * m + j, L[j]);
+    }
 }
 
-// strided lines: lines q in [0, nlines), element j at base(q) + j * stride,
-// base(q) = (q / inner) * outer + q % inner. Lane per line, W waves per
-// workgroup split the line into segments of SEG.
+// strided lines along dim d (1 or 0) of the coarse grid, lane per line.
+// Lines are indexed over padded rows, q = r * fp + j2 (r = j0 for d = 1,
+// j1 for d = 0); lanes with j2 >= c2 idle, so each wave's 64 accesses are
+// one aligned 512-byte run of f. W waves per workgroup split the line into
+// segments of SEG.
 template <class T, int SEG>
-__global__ __launch_bounds__(512) void k_solve_strided(SolveParams<T> S, uint32_t m, uint64_t nlines, uint64_t inner,
-                                                        uint64_t outer, uint64_t stride) {
+__global__ __launch_bounds__(512) void k_solve_strided(SolveParams<T> S, uint32_t m, uint64_t nlines, int d) {
     __shared__ double sA[8][64], sB[8][64];
     const uint32_t lane = lane_id();
     const uint32_t wv = threadIdx.x / 64, nw = blockDim.x / 64;
     const uint64_t q = (uint64_t)blockIdx.x * 64 + lane;
-    const bool act = q < nlines;
-    const uint64_t base = act ? (q / inner) * outer + (q % inner) : 0;
+    const uint32_t fp = S.fp, c1 = S.c[1], c2 = S.c[2];
+    const uint32_t j2 = (uint32_t)(q % fp);
+    const uint64_t r = q / fp;
+    const bool act = q < nlines && j2 < c2;
+    uint64_t base = 0, stride, ubase = 0, ustride;
+    if (d == 1) {
+        stride = fp;
+        ustride = c2;
+        if (act) { base = r * c1 * fp + j2; ubase = r * c1 * c2 + j2; }
+    } else {
+        stride = (uint64_t)c1 * fp;
+        ustride = (uint64_t)c1 * c2;
+        if (act) { base = r * fp + j2; ubase = r * c2 + j2; }
+    }
     const uint32_t seg = (m + nw - 1) / nw;
     // wave-uniform segment bounds: the factor loads become scalar loads
     const uint32_t b = (uint32_t)__builtin_amdgcn_readfirstlane((int)(wv * seg));
@@ -574,7 +611,7 @@ __global__ __launch_bounds__(512) void k_solve_strided(SolveParams<T> S, uint32_
     if (act) {
 #pragma unroll
         for (int i = 0; i < SEG; i++)
-            if (b + i < e) epilogue(S, base + (uint64_t)(b + i) * stride, v[i]);
+            if (b + i < e) epilogue(S, base + (uint64_t)(b + i) * stride, ubase + (uint64_t)(b + i) * ustride, v[i]);
     }
 }
 
@@ -725,7 +762,8 @@ WsLayout ws_layout(const LevelGeom &g, size_t elem) {
     w.xa = 0;
     w.xb = al(w.xa + p1 * elem);
     w.f = al(w.xb + std::max(p1, p2) * elem);
-    w.tab = al(w.f + p1 * sizeof(double));
+    const uint64_t f1 = g.L >= 1 ? (uint64_t)g.ld[g.L - 1][0] * g.ld[g.L - 1][1] * f_pitch(g.ld[g.L - 1][2]) : 1;
+    w.tab = al(w.f + f1 * sizeof(double));
     size_t tab = 0;
     for (int l = 1; l <= g.L; l++)
         for (int d = 0; d < 3; d++) tab += 2 * g.ld[l - 1][d];
@@ -746,6 +784,7 @@ int solve_all(const LevelGeom &g, int l, double *f, const double *tab_dev, const
         if (n[d] == 1) continue;
         SolveParams<T> S;
         S.f = f;
+        S.fp = f_pitch(c[2]);
         S.c[0] = c[0]; S.c[1] = c[1]; S.c[2] = c[2];
         S.inv = tab_dev + F.off[l][d];
         S.cp = S.inv + c[d];
@@ -762,24 +801,22 @@ int solve_all(const LevelGeom &g, int l, double *f, const double *tab_dev, const
             else if (seg <= 16) hipLaunchKernelGGL((k_solve_contig<T, 16>), dim3(grid), dim3(256), 0, s, S, nlines);
             else { set_error("line too long for the contiguous solve"); return kErrUnsupported; }
         } else {
-            uint64_t nlines, inner, outer, stride;
-            if (d == 1) { nlines = (uint64_t)c[0] * c[2]; inner = c[2]; outer = (uint64_t)c[1] * c[2]; stride = c[2]; }
-            else { nlines = (uint64_t)c[1] * c[2]; inner = nlines; outer = 0; stride = (uint64_t)c[1] * c[2]; }
+            const uint64_t nlines = (uint64_t)(d == 1 ? c[0] : c[1]) * S.fp;
             // up to 8 waves split a line; each lane keeps its segment in registers
             uint32_t nw = std::min<uint32_t>(8, (m + 15) / 16);
             if (nw < 1) nw = 1;
             const uint32_t seg = (m + nw - 1) / nw;
             const uint32_t grid = div_up(nlines, 64);
             if (seg <= 8)
-                hipLaunchKernelGGL((k_solve_strided<T, 8>), dim3(grid), dim3(64 * nw), 0, s, S, m, nlines, inner, outer, stride);
+                hipLaunchKernelGGL((k_solve_strided<T, 8>), dim3(grid), dim3(64 * nw), 0, s, S, m, nlines, d);
             else if (seg <= 16)
-                hipLaunchKernelGGL((k_solve_strided<T, 16>), dim3(grid), dim3(64 * nw), 0, s, S, m, nlines, inner, outer, stride);
+                hipLaunchKernelGGL((k_solve_strided<T, 16>), dim3(grid), dim3(64 * nw), 0, s, S, m, nlines, d);
             else if (seg <= 24)
-                hipLaunchKernelGGL((k_solve_strided<T, 24>), dim3(grid), dim3(64 * nw), 0, s, S, m, nlines, inner, outer, stride);
+                hipLaunchKernelGGL((k_solve_strided<T, 24>), dim3(grid), dim3(64 * nw), 0, s, S, m, nlines, d);
             else if (seg <= 40)
-                hipLaunchKernelGGL((k_solve_strided<T, 40>), dim3(grid), dim3(64 * nw), 0, s, S, m, nlines, inner, outer, stride);
+                hipLaunchKernelGGL((k_solve_strided<T, 40>), dim3(grid), dim3(64 * nw), 0, s, S, m, nlines, d);
             else if (seg <= 72)
-                hipLaunchKernelGGL((k_solve_strided<T, 72>), dim3(grid), dim3(64 * nw), 0, s, S, m, nlines, inner, outer, stride);
+                hipLaunchKernelGGL((k_solve_strided<T, 72>), dim3(grid), dim3(64 * nw), 0, s, S, m, nlines, d);
             else { set_error("line too long for the strided solve"); return kErrUnsupported; }
         }
         PRODM_CHECK_LAUNCH();
@@ -853,6 +890,7 @@ int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g,
         T *uc = bufs[which];
         P.uc = uc;
         P.f = f;
+        P.fp = f_pitch(P.c[2]);
         P.key = level_keys ? level_keys + l : nullptr;
         int rc;
         {
@@ -920,6 +958,7 @@ int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int tables_ready
     for (int l = 1; l <= g.L; l++) {
         LoadParams<T> P = load_params(g, l, src);
         P.f = f;
+        P.fp = f_pitch(P.c[2]);
         {
             StageScope sc(tm, l == g.L ? kStRecLoadTop : kStRecLoadRest);
             rc = launch_load<T, kRecompose>(P, ilv, s);
@@ -967,12 +1006,16 @@ int mgard_adjoint_coarse(const double *gfine, const LevelGeom &g, int l, double 
     P.is[1] = P.n[2];
     P.is[2] = 1;
     P.f = f;
+    P.fp = f_pitch(P.c[2]);
     int rc = launch_load<double, kAdjointPT>(P, false, s);
     if (rc) return rc;
-    PRODM_HIP(hipMemcpyAsync(h_out, f, points3(g.ld[l - 1]) * sizeof(double), hipMemcpyDeviceToDevice, s));
+    const uint32_t *c = g.ld[l - 1];
+    rc = copy_box<double>(f, (uint64_t)c[1] * P.fp, P.fp, h_out, (uint64_t)c[1] * c[2], c[2], c, s);
+    if (rc) return rc;
     return solve_all<double>(g, l, f, tab, F, kEpiNone, nullptr, nullptr, s);
 }
-const double *mgard_adjoint_solution(const LevelGeom &g, void *ws) {
+const double *mgard_adjoint_solution(const LevelGeom &g, int l, void *ws, uint32_t *pitch) {
+    *pitch = f_pitch(g.ld[l - 1][2]);
     return reinterpret_cast<const double *>(static_cast<char *>(ws) + ws_layout(g, sizeof(double)).f);
 }
 
