@@ -140,6 +140,7 @@ def main():
     ap.add_argument("--bitplanes", type=int, default=32)
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="time HIP graph replays instead of eager launches")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only a plain copy kernel of known traffic (for counter scaling) and exit")
     args = ap.parse_args()
@@ -193,17 +194,46 @@ def main():
     torch.cuda.synchronize()
 
     # timed region: exactly K steps of refactor + retrieve, nothing else on
-    # the stream, bracketed by synchronizations
-    s = torch.cuda.current_stream()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.refactor(u)
-        plan.retrieve(planes, out)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    # the stream, bracketed by synchronizations. Eager launches by default;
+    # --graph replays the same kernels from captured HIP graphs.
+    gs = torch.cuda.Stream()
+    with torch.cuda.stream(gs):
+        if not (not args.graph):
+            for _ in range(max(1, args.warmup)):
+                plan.refactor_graph(u)
+                plan.retrieve_graph(planes, out)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if (not args.graph):
+            for _ in range(args.steps):
+                plan.refactor(u)
+                plan.retrieve(planes, out)
+        else:
+            for _ in range(args.steps):
+                plan.refactor_graph(u)
+                plan.retrieve_graph(planes, out)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
     ms_step = wall * 1e3 / args.steps
     value = 2 * in_bytes / (ms_step * 1e6)
+    err_graph = float(torch.max(torch.abs(out.double() - u.double())))
+    if not err_graph <= tau:
+        log("[bench] FAIL: timed retrieval misses the tolerance")
+        sys.exit(1)
+    s = torch.cuda.current_stream()
+    # the other mode, for the record
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.cuda.stream(gs):
+        for _ in range(args.steps):
+            if (not args.graph):
+                plan.refactor_graph(u)
+                plan.retrieve_graph(planes, out)
+            else:
+                plan.refactor(u)
+                plan.retrieve(planes, out)
+        torch.cuda.synchronize()
+    other_ms = (time.perf_counter() - t0) * 1e3 / args.steps
 
     # refactor / retrieve split: K more steps with one event between the two
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
@@ -280,6 +310,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_step, 4),
+        "launch_mode": "eager" if (not args.graph) else "hip_graph",
+        ("graph_ms_per_step" if (not args.graph) else "eager_ms_per_step"): round(other_ms, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

@@ -173,6 +173,14 @@ int prodm_mdr_level_coefficients(prodm_mdr_plan *plan, int level, void **d_ptr);
 /* Retrieve planes [0, planes[l]) of every level from the plan's plane
  * storage and recompose to the full grid (d_out dense).                   */
 int prodm_mdr_retrieve(prodm_mdr_plan *plan, const uint8_t *planes_per_level, void *d_out, prodm_stream_t stream);
+/* Graph forms of prodm_mdr_refactor / prodm_mdr_retrieve (no counterpart
+ * in ProDM, whose path is host code): the first call for a given input
+ * pointer (refactor) or (planes per level, output pointer) (retrieve)
+ * captures the whole kernel sequence on `stream` into a HIP graph, later
+ * calls replay it with one launch. The stream must not be the default
+ * stream. Results are identical to the eager calls.                      */
+int prodm_mdr_refactor_graph(prodm_mdr_plan *plan, const void *d_in, prodm_stream_t stream);
+int prodm_mdr_retrieve_graph(prodm_mdr_plan *plan, const uint8_t *planes_per_level, void *d_out, prodm_stream_t stream);
 /* Reference-format byte stream of one plane, copied to host memory
  * (NegaBinary: ceil(n/B) words, NegaBinaryBPEncoder.hpp:241-249; GroupedBP
  * plane 0 carries the u32 nblocks + u8 rb[nblocks] prefix,

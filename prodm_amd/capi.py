@@ -64,6 +64,8 @@ _SIGS = {
     "prodm_mdr_retrieve": (C.c_int, [vp, vp, vp, vp]),
     "prodm_mdr_profile": (C.c_int, [vp, C.c_int]),
     "prodm_mdr_stage_times": (C.c_int, [vp, vp, vp, C.c_int]),
+    "prodm_mdr_refactor_graph": (C.c_int, [vp, vp, vp]),
+    "prodm_mdr_retrieve_graph": (C.c_int, [vp, vp, vp, vp]),
     "prodm_mdr_plane_stream": (C.c_int, [vp, C.c_int, C.c_int, vp, u64, vp, vp]),
     "prodm_mdr_load_plane_stream": (C.c_int, [vp, C.c_int, C.c_int, vp, u64, vp]),
     "prodm_mdr_set_level_max": (C.c_int, [vp, C.c_int, C.c_double, vp]),

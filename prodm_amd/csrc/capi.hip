@@ -276,6 +276,12 @@ struct prodm_mdr_plan {
     size_t gbws_bytes = 0;
     uint64_t device_bytes = 0;
     prodm::StageTimer timer;
+    // captured graphs: refactor of one input pointer, retrieve of one
+    // (planes per level, output pointer) pair
+    hipGraphExec_t g_ref = nullptr, g_ret = nullptr;
+    const void *g_ref_in = nullptr;
+    void *g_ret_out = nullptr;
+    uint8_t g_ret_k[33] = {0};
 };
 
 namespace prodm {
@@ -289,6 +295,8 @@ static int plan_alloc(void **p, size_t bytes, uint64_t &total) {
 }
 
 static void plan_free(prodm_mdr_plan *p) {
+    if (p->g_ref) (void)hipGraphExecDestroy(p->g_ref);
+    if (p->g_ret) (void)hipGraphExecDestroy(p->g_ret);
     void *ptrs[] = {p->ws, p->levels, p->keys, p->planes, p->rb, p->words, p->signs, p->gbws};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
@@ -727,6 +735,53 @@ int prodm_mdr_set_level_max(prodm_mdr_plan *p, int level, double max_abs, prodm_
     uint64_t k;
     std::memcpy(&k, &max_abs, 8);
     return launch_fill_u64(reinterpret_cast<uint64_t *>(p->keys) + level, 1, k, s);
+}
+
+// Capture `body` on stream s into a graph executable (kernel nodes only).
+extern "C++" template <class F> static int capture(hipStream_t s, hipGraphExec_t *exec, F body) {
+    if (!s) { set_error("graph capture needs a non-default stream"); return kErrArg; }
+    if (*exec) { (void)hipGraphExecDestroy(*exec); *exec = nullptr; }
+    PRODM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = body();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) { set_error(std::string("graph capture: ") + hipGetErrorString(e)); return kErrHip; }
+    const hipError_t ei = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) { *exec = nullptr; set_error(std::string("graph instantiate: ") + hipGetErrorString(ei)); return kErrHip; }
+    return kOk;
+}
+
+int prodm_mdr_refactor_graph(prodm_mdr_plan *p, const void *d_in, prodm_stream_t stream) {
+    if (!p || !d_in) { set_error("refactor_graph: null argument"); return kErrArg; }
+    hipStream_t s = (hipStream_t)stream;
+    if (!p->g_ref || p->g_ref_in != d_in) {
+        const bool on = p->timer.on;
+        p->timer.on = false;  // no event nodes in the graph
+        const int rc = capture(s, &p->g_ref, [&] { return prodm_mdr_refactor(p, d_in, stream); });
+        p->timer.on = on;
+        if (rc) return rc;
+        p->g_ref_in = d_in;
+    }
+    PRODM_HIP(hipGraphLaunch(p->g_ref, s));
+    return kOk;
+}
+
+int prodm_mdr_retrieve_graph(prodm_mdr_plan *p, const uint8_t *k, void *d_out, prodm_stream_t stream) {
+    if (!p || !k || !d_out) { set_error("retrieve_graph: null argument"); return kErrArg; }
+    hipStream_t s = (hipStream_t)stream;
+    if (!p->g_ret || p->g_ret_out != d_out || std::memcmp(p->g_ret_k, k, p->g.L + 1) != 0) {
+        const bool on = p->timer.on;
+        p->timer.on = false;
+        const int rc = capture(s, &p->g_ret, [&] { return prodm_mdr_retrieve(p, k, d_out, stream); });
+        p->timer.on = on;
+        if (rc) return rc;
+        p->g_ret_out = d_out;
+        std::memcpy(p->g_ret_k, k, p->g.L + 1);
+    }
+    PRODM_HIP(hipGraphLaunch(p->g_ret, s));
+    return kOk;
 }
 
 int prodm_mdr_retrieve(prodm_mdr_plan *p, const uint8_t *k, void *d_out, prodm_stream_t stream) {

@@ -913,7 +913,10 @@ int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g,
     const uint32_t *c0 = g.ld[0];
     if (ilv) {
         if (g.L == 0) return copy_box<T>(in, in_strides[0], in_strides[1], sink.level[0], (uint64_t)c0[1] * c0[2], c0[2], c0, s);
-        PRODM_HIP(hipMemcpyAsync(sink.level[0], cur, points3(c0) * sizeof(T), hipMemcpyDeviceToDevice, s));
+        // a kernel rather than a copy node, so that the sequence captures
+        // into a graph as kernel nodes only
+        int rc = copy_box<T>(cur, cs[0], cs[1], sink.level[0], (uint64_t)c0[1] * c0[2], c0[2], c0, s);
+        if (rc) return rc;
         if (level_keys) return launch_maxabs<T>(sink.level[0], points3(c0), level_keys + 0, s);
         return kOk;
     }

@@ -275,6 +275,16 @@ class Plan:
         check(lib().prodm_mdr_retrieve(self.h, k, _ptr(out), _stream()), "retrieve")
         return out
 
+    def refactor_graph(self, x):
+        """refactor through a captured HIP graph (current stream must not be
+        the default stream)."""
+        check(lib().prodm_mdr_refactor_graph(self.h, _ptr(x), _stream()), "refactor_graph")
+
+    def retrieve_graph(self, planes_per_level, out):
+        k = (C.c_uint8 * (self.L + 1))(*[int(v) for v in planes_per_level])
+        check(lib().prodm_mdr_retrieve_graph(self.h, k, _ptr(out), _stream()), "retrieve_graph")
+        return out
+
     def plane_stream(self, level, p):
         """Reference-format byte stream of plane p of a level (host bytes)."""
         n = C.c_uint64()
