@@ -452,27 +452,17 @@ template <class T> __device__ inline void epilogue(const SolveParams<T> &S, uint
 // contiguous lines (dim 2): one wave per line, lane holds a segment of SEG
 template <class T, int SEG>
 __global__ __launch_bounds__(256) void k_solve_contig(SolveParams<T> S, uint64_t nlines) {
-    // the line moves between memory and LDS with coalesced 64-lane accesses;
-    // each lane then solves its contiguous segment from LDS
-    __shared__ double buf[4][64 * SEG];
     const uint32_t m = S.c[2];
     const uint32_t lane = lane_id(), wv = threadIdx.x / 64;
     const uint64_t line = (uint64_t)blockIdx.x * 4 + wv;
     if (line >= nlines) return;
-    double *L = buf[wv];
     const double *x = S.f + line * S.fp;
-#pragma unroll
-    for (int k = 0; k < SEG; k++) {
-        const uint32_t j = k * 64 + lane;
-        if (j < m) L[j] = x[j];
-    }
-    __builtin_amdgcn_wave_barrier();
     const uint32_t seg = (m + 63) / 64;
     const uint32_t b = lane * seg;
     const uint32_t e = min(m, b + seg);
     double v[SEG];
 #pragma unroll
-    for (int i = 0; i < SEG; i++) v[i] = (b + i < e) ? L[b + i] : 0.0;
+    for (int i = 0; i < SEG; i++) v[i] = (b + i < e) ? x[b + i] : 0.0;
     // forward: y_i = (f_i - y_{i-1}) * inv_i  == affine map y_{i-1} -> y_i
     double A = 0.0, Bm = 1.0;
 #pragma unroll
@@ -520,16 +510,9 @@ __global__ __launch_bounds__(256) void k_solve_contig(SolveParams<T> S, uint64_t
             v[i] = z;
         }
     }
-    __builtin_amdgcn_wave_barrier();
 #pragma unroll
     for (int i = 0; i < SEG; i++)
-        if (b + i < e) L[b + i] = v[i];
-    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-    for (int k = 0; k < SEG; k++) {
-        const uint32_t j = k * 64 + lane;
-        if (j < m) epilogue(S, line * S.fp + j, line * m + j, L[j]);
-    }
+        if (b + i < e) epilogue(S, line * S.fp + b + i, line * m + b + i, v[i]);
 }
 
 // strided lines along dim d (1 or 0) of the coarse grid, lane per line.
