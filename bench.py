@@ -148,8 +148,8 @@ def main():
         calibrate_copy()
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU layout is described in "
-              "DESIGN.md and is not run.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}): this build runs on a single GPU and "
+              "has no multi-GPU path.", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -6,6 +6,7 @@
 // composed.hpp) in place of the CPU class:
 //
 //   GpuMGARDOrthogonalDecomposer<T>  MDR::MGARDOrthoganalDecomposer<T>  (Decomposer/MGARD.hpp:10-40)
+//   GpuMGARDHierarchicalDecomposer<T> MDR::MGARDHierarchicalDecomposer<T> (Decomposer/MGARD.hpp:42-70)
 //   GpuDirectInterleaver<T>          MDR::DirectInterleaver<T>          (Interleaver/DirectInterleaver.hpp:9-67)
 //   GpuNegaBinaryBPEncoder<T, S>     MDR::NegaBinaryBPEncoder<T, S>     (BitplaneEncoder/NegaBinaryBPEncoder.hpp:12-262)
 //   GpuGroupedBPEncoder<T, S>        MDR::GroupedBPEncoder<T, S>        (BitplaneEncoder/GroupedBPEncoder.hpp:12-380)
@@ -32,10 +33,12 @@
 namespace prodm {
 
 // ---------------------------------------------------------------- decomposer
-template <class T>
-class GpuMGARDOrthogonalDecomposer {
+namespace detail {
+// HIER: hierarchical basis (MGARDHierarchicalDecomposer), else orthogonal
+template <class T, bool HIER>
+class GpuMGARDDecomposer {
   public:
-    explicit GpuMGARDOrthogonalDecomposer(Context ctx = make_context()) : ctx_(std::move(ctx)) {}
+    explicit GpuMGARDDecomposer(Context ctx = make_context()) : ctx_(std::move(ctx)) {}
     void decompose(T *data, const std::vector<uint32_t> &dims, uint32_t target_level,
                    std::vector<uint32_t> strides = std::vector<uint32_t>()) const {
         run(true, data, dims, target_level, strides);
@@ -44,7 +47,9 @@ class GpuMGARDOrthogonalDecomposer {
                    std::vector<uint32_t> strides = std::vector<uint32_t>()) const {
         run(false, data, dims, target_level, strides);
     }
-    void print() const { std::printf("MGARD orthogonal decomposer (MI355X)\n"); }
+    void print() const {
+        std::printf(HIER ? "MGARD hierarchical decomposer (MI355X)\n" : "MGARD orthogonal decomposer (MI355X)\n");
+    }
 
   private:
     void run(bool forward, T *data, const std::vector<uint32_t> &dims, uint32_t L,
@@ -61,16 +66,37 @@ class GpuMGARDOrthogonalDecomposer {
         void *ws = c.ws.get(wsb);
         const uint32_t *st = strides.empty() ? nullptr : strides.data();
         int rc;
-        if constexpr (std::is_same<T, float>::value)
-            rc = forward ? prodm_decompose_f32(a, b, dims.data(), nd, (int)L, st, ws, wsb, c.stream)
-                         : prodm_recompose_f32(a, b, dims.data(), nd, (int)L, st, ws, wsb, c.stream);
-        else
-            rc = forward ? prodm_decompose_f64(a, b, dims.data(), nd, (int)L, st, ws, wsb, c.stream)
-                         : prodm_recompose_f64(a, b, dims.data(), nd, (int)L, st, ws, wsb, c.stream);
+        if constexpr (std::is_same<T, float>::value) {
+            if (HIER)
+                rc = forward ? prodm_decompose_hierarchical_f32(a, b, dims.data(), nd, (int)L, st, ws, wsb, c.stream)
+                             : prodm_recompose_hierarchical_f32(a, b, dims.data(), nd, (int)L, st, ws, wsb, c.stream);
+            else
+                rc = forward ? prodm_decompose_f32(a, b, dims.data(), nd, (int)L, st, ws, wsb, c.stream)
+                             : prodm_recompose_f32(a, b, dims.data(), nd, (int)L, st, ws, wsb, c.stream);
+        } else {
+            if (HIER)
+                rc = forward ? prodm_decompose_hierarchical_f64(a, b, dims.data(), nd, (int)L, st, ws, wsb, c.stream)
+                             : prodm_recompose_hierarchical_f64(a, b, dims.data(), nd, (int)L, st, ws, wsb, c.stream);
+            else
+                rc = forward ? prodm_decompose_f64(a, b, dims.data(), nd, (int)L, st, ws, wsb, c.stream)
+                             : prodm_recompose_f64(a, b, dims.data(), nd, (int)L, st, ws, wsb, c.stream);
+        }
         check(rc, forward ? "decompose" : "recompose");
         to_host(data, b, n, c.stream);
     }
     Context ctx_;
+};
+}  // namespace detail
+
+template <class T>
+class GpuMGARDOrthogonalDecomposer : public detail::GpuMGARDDecomposer<T, false> {
+  public:
+    using detail::GpuMGARDDecomposer<T, false>::GpuMGARDDecomposer;
+};
+template <class T>
+class GpuMGARDHierarchicalDecomposer : public detail::GpuMGARDDecomposer<T, true> {
+  public:
+    using detail::GpuMGARDDecomposer<T, true>::GpuMGARDDecomposer;
 };
 
 // ---------------------------------------------------------------- interleaver

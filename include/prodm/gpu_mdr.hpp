@@ -51,9 +51,12 @@ class Plan {
 template <class T, class Compressor, class Writer>
 class GpuRefactor {
   public:
-    // encoder: PRODM_ENC_NEGABINARY (ProDM's `negabinary = true`) or PRODM_ENC_GROUPED.
-    GpuRefactor(int encoder, Compressor compressor, Writer writer, bool collect_squared = false)
-        : encoder_(encoder), compressor_(compressor), writer_(writer), squared_(collect_squared) {}
+    // encoder: PRODM_ENC_NEGABINARY (ProDM's `negabinary = true`) or PRODM_ENC_GROUPED;
+    // decomposer: PRODM_DECOMP_ORTHOGONAL or PRODM_DECOMP_HIERARCHICAL.
+    GpuRefactor(int encoder, Compressor compressor, Writer writer, bool collect_squared = false,
+                int decomposer = PRODM_DECOMP_ORTHOGONAL)
+        : encoder_(encoder), compressor_(compressor), writer_(writer), squared_(collect_squared),
+          decomposer_(decomposer) {}
 
     // Host input (copied to the device) ...
     void refactor(T const *data, const std::vector<uint32_t> &dims, uint8_t target_level, uint8_t num_bitplanes) {
@@ -104,6 +107,7 @@ class GpuRefactor {
         if (plan_ && dims == dims_ && L == L_ && nb == nb_) return;
         plan_.reset();
         plan_ = std::make_shared<Plan>(dims, L, nb, dtype_of<T>(), encoder_);
+        check(prodm_mdr_plan_set_decomposer(plan_->get(), decomposer_), "plan decomposer");
         dims_ = dims;
         L_ = L;
         nb_ = nb;
@@ -126,6 +130,7 @@ class GpuRefactor {
     Compressor compressor_;
     Writer writer_;
     bool squared_;
+    int decomposer_;
     hipStream_t stream_ = nullptr;
     std::shared_ptr<Plan> plan_;
     std::vector<uint32_t> dims_;
@@ -136,8 +141,11 @@ class GpuRefactor {
 template <class T, class Compressor, class SizeInterpreter, class ErrorEstimator, class Retriever>
 class GpuReconstructor {
   public:
-    GpuReconstructor(Compressor compressor, SizeInterpreter interpreter, Retriever retriever)
-        : compressor_(compressor), interpreter_(interpreter), retriever_(retriever) {}
+    // decomposer: the one the files were refactored with (ProDM's metadata
+    // does not record it)
+    GpuReconstructor(Compressor compressor, SizeInterpreter interpreter, Retriever retriever,
+                     int decomposer = PRODM_DECOMP_ORTHOGONAL)
+        : compressor_(compressor), interpreter_(interpreter), retriever_(retriever), decomposer_(decomposer) {}
 
     void load_metadata() {
         size_t size = 0;
@@ -150,6 +158,7 @@ class GpuReconstructor {
             if ((int)s.size() != nb) fatal("reconstruct: levels with different plane counts");
         plan_ = std::make_shared<Plan>(meta_.dims, L, nb, dtype_of<T>(),
                                        meta_.negabinary ? PRODM_ENC_NEGABINARY : PRODM_ENC_GROUPED);
+        check(prodm_mdr_plan_set_decomposer(plan_->get(), decomposer_), "plan decomposer");
         for (int l = 0; l <= L; l++) {
             const double b = meta_.negabinary ? (double)(meta_.level_error_bounds[l] / 4) : (double)meta_.level_error_bounds[l];
             check(prodm_mdr_set_level_max(plan_->get(), l, b, stream_), "set level max");
@@ -231,6 +240,7 @@ class GpuReconstructor {
     Compressor compressor_;
     SizeInterpreter interpreter_;
     Retriever retriever_;
+    int decomposer_;
     MdrMetadata<T> meta_;
     std::shared_ptr<Plan> plan_;
     std::vector<uint8_t> held_;

@@ -60,6 +60,20 @@ int prodm_recompose_f32(const float *d_in, float *d_out, const uint32_t *dims, i
 int prodm_recompose_f64(const double *d_in, double *d_out, const uint32_t *dims, int ndims, int target_level,
                         const uint32_t *strides, void *d_ws, size_t ws_bytes, prodm_stream_t stream);
 
+/* ---- Decomposer: MGARDHierarchicalDecomposer<T>::decompose / recompose
+ *      (include/MDR/Decomposer/MGARD.hpp:42-70): the same level steps
+ *      without the L2 correction. Same arguments and layout as above.     */
+int prodm_decompose_hierarchical_f32(const float *d_in, float *d_out, const uint32_t *dims, int ndims, int target_level,
+                                     const uint32_t *strides, void *d_ws, size_t ws_bytes, prodm_stream_t stream);
+int prodm_decompose_hierarchical_f64(const double *d_in, double *d_out, const uint32_t *dims, int ndims,
+                                     int target_level, const uint32_t *strides, void *d_ws, size_t ws_bytes,
+                                     prodm_stream_t stream);
+int prodm_recompose_hierarchical_f32(const float *d_in, float *d_out, const uint32_t *dims, int ndims, int target_level,
+                                     const uint32_t *strides, void *d_ws, size_t ws_bytes, prodm_stream_t stream);
+int prodm_recompose_hierarchical_f64(const double *d_in, double *d_out, const uint32_t *dims, int ndims,
+                                     int target_level, const uint32_t *strides, void *d_ws, size_t ws_bytes,
+                                     prodm_stream_t stream);
+
 /* ---- Adjoint (transpose) of recompose, FP64 inside. No counterpart in
  *      ProDM (whose reconstructor only applies recompose,
  *      ComposedReconstructor.hpp:214,250-253); it gives the sensitivity of
@@ -173,6 +187,11 @@ int prodm_mdr_level_coefficients(prodm_mdr_plan *plan, int level, void **d_ptr);
 /* Retrieve planes [0, planes[l]) of every level from the plan's plane
  * storage and recompose to the full grid (d_out dense).                   */
 int prodm_mdr_retrieve(prodm_mdr_plan *plan, const uint8_t *planes_per_level, void *d_out, prodm_stream_t stream);
+/* Decomposer of a plan: orthogonal (default, MGARDOrthoganalDecomposer,
+ * Decomposer/MGARD.hpp:10-40) or hierarchical (MGARD.hpp:42-70).           */
+#define PRODM_DECOMP_ORTHOGONAL 0
+#define PRODM_DECOMP_HIERARCHICAL 1
+int prodm_mdr_plan_set_decomposer(prodm_mdr_plan *plan, int kind);
 /* Graph forms of prodm_mdr_refactor / prodm_mdr_retrieve (no counterpart
  * in ProDM, whose path is host code): the first call for a given input
  * pointer (refactor) or (planes per level, output pointer) (retrieve)

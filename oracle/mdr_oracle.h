@@ -102,6 +102,12 @@ int orc_decompose_f32(float *data, const uint32_t *dims, int nd, int L, const ui
 int orc_decompose_f64(double *data, const uint32_t *dims, int nd, int L, const uint32_t *strides);
 int orc_recompose_f32(float *data, const uint32_t *dims, int nd, int L, const uint32_t *strides);
 int orc_recompose_f64(double *data, const uint32_t *dims, int nd, int L, const uint32_t *strides);
+/* Hierarchical basis (MGARDHierarchicalDecomposer, Decomposer/MGARD.hpp:42-70):
+ * the same steps without the L2 correction.                              */
+int orc_decompose_hb_f32(float *data, const uint32_t *dims, int nd, int L, const uint32_t *strides);
+int orc_decompose_hb_f64(double *data, const uint32_t *dims, int nd, int L, const uint32_t *strides);
+int orc_recompose_hb_f32(float *data, const uint32_t *dims, int nd, int L, const uint32_t *strides);
+int orc_recompose_hb_f64(double *data, const uint32_t *dims, int nd, int L, const uint32_t *strides);
 /* Adjoint (transpose) of recompose as a linear map on FP64 arrays:
  * given g = dJ/du (full grid), returns dJ/dcoef in the decomposed layout.  */
 int orc_recompose_adjoint_f64(double *data, const uint32_t *dims, int nd, int L,

@@ -2,13 +2,16 @@
 // (counterpart of ProDM's test/test_mdr_reconstructor.cpp:44-88).
 //
 //   prodm_reconstruct <original data file> <num tolerances> <tolerances...>
-//                     [--f64] [--l2] [--in DIR]
+//                     [--f64] [--l2] [--hierarchical] [--in DIR]
 //
 // Reads DIR/metadata.bin (default DIR refactored_data). For each tolerance:
 // plans with the greedy interpreter (L-infinity: MaxErrorEstimatorOB;
 // --l2: S-norm estimator with the negabinary greedy interpreter, needs files
-// written with --squared), reads and decodes the new planes, recomposes, and
-// prints the retrieved bytes and the error against the original.
+// written with --squared; --hierarchical: files written with the hierarchical
+// basis, planned with MaxErrorEstimatorHB and the sign-excluding greedy
+// planner as ProDM's driver does), reads and decodes the new planes,
+// recomposes, and prints the retrieved bytes and the error against the
+// original.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -55,7 +58,7 @@ static void evaluate(const std::vector<T> &u, const std::vector<double> &tols, R
 }
 
 template <class T>
-static int run(const std::string &file, const std::vector<double> &tols, bool l2, const std::string &dir) {
+static int run(const std::string &file, const std::vector<double> &tols, bool l2, bool hier, const std::string &dir) {
     size_t n = 0;
     const auto u = read_raw<T>(file, n);
     size_t msize = 0;
@@ -69,7 +72,15 @@ static int run(const std::string &file, const std::vector<double> &tols, bool l2
         for (int l = 0; l < nl; l++) files.push_back(dir + "/level_" + std::to_string(l) + ".bin");
     }
     ConcatLevelFileRetriever retriever(dir + "/metadata.bin", files);
-    if (l2) {
+    if (hier) {
+        using E = MaxErrorEstimatorHB<T>;
+        GpuReconstructor<T, AdaptiveLevelCompressor, SignExcludeGreedyBasedSizeInterpreter<E>, E,
+                         ConcatLevelFileRetriever>
+            r(AdaptiveLevelCompressor(64), SignExcludeGreedyBasedSizeInterpreter<E>(E()), retriever,
+              PRODM_DECOMP_HIERARCHICAL);
+        r.load_metadata();
+        evaluate(u, tols, r);
+    } else if (l2) {
         using E = SNormErrorEstimator<T>;
         const int L = (int)files.size() - 1;
         GpuReconstructor<T, AdaptiveLevelCompressor, NegaBinaryGreedyBasedSizeInterpreter<E>, E, ConcatLevelFileRetriever>
@@ -97,14 +108,15 @@ int main(int argc, char **argv) {
     const int nt = std::atoi(argv[a++]);
     std::vector<double> tols;
     for (int i = 0; i < nt && a < argc; i++) tols.push_back(std::atof(argv[a++]));
-    bool f64 = false, l2 = false;
+    bool f64 = false, l2 = false, hier = false;
     std::string dir = "refactored_data";
     for (; a < argc; a++) {
         const std::string o = argv[a];
         if (o == "--f64") f64 = true;
         else if (o == "--l2") l2 = true;
+        else if (o == "--hierarchical") hier = true;
         else if (o == "--in" && a + 1 < argc) dir = argv[++a];
         else fatal("unknown option " + o);
     }
-    return f64 ? run<double>(file, tols, l2, dir) : run<float>(file, tols, l2, dir);
+    return f64 ? run<double>(file, tols, l2, hier, dir) : run<float>(file, tols, l2, hier, dir);
 }

@@ -2,12 +2,13 @@
 // (counterpart of ProDM's test/test_mdr_refactor.cpp:34-86).
 //
 //   prodm_refactor <data file> <target level> <num bitplanes> <num dims> <dims...>
-//                  [--f64] [--grouped] [--composed] [--squared] [--out DIR]
+//                  [--f64] [--grouped] [--composed] [--squared] [--hierarchical] [--out DIR]
 //
 // Writes DIR/metadata.bin and DIR/level_<l>.bin (default DIR refactored_data).
 // Default: float, negabinary encoder, AdaptiveLevelCompressor(64), the
 // device-resident GpuRefactor; --composed drives the GPU components through
-// prodm::ComposedRefactor instead (same files).
+// prodm::ComposedRefactor instead (same files). --hierarchical uses the
+// hierarchical basis (ProDM's drivers use MGARDHierarchicalDecomposer).
 #include <sys/stat.h>
 
 #include <chrono>
@@ -31,7 +32,7 @@ static std::vector<T> read_raw(const std::string &path, size_t n) {
 
 template <class T, class S>
 static int run(const std::string &file, const std::vector<uint32_t> &dims, int L, int nb, bool grouped, bool composed,
-               bool squared, const std::string &dir) {
+               bool squared, bool hier, const std::string &dir) {
     const auto data = read_raw<T>(file, product(dims));
     std::vector<std::string> files;
     for (int l = 0; l <= L; l++) files.push_back(dir + "/level_" + std::to_string(l) + ".bin");
@@ -39,13 +40,16 @@ static int run(const std::string &file, const std::vector<uint32_t> &dims, int L
     const auto t0 = std::chrono::steady_clock::now();
     if (composed) {
         Context ctx = make_context();
-        auto go = [&](auto encoder, auto collector) {
-            ComposedRefactor<T, GpuMGARDOrthogonalDecomposer<T>, GpuDirectInterleaver<T>, decltype(encoder),
+        auto go2 = [&](auto decomposer, auto encoder, auto collector) {
+            ComposedRefactor<T, decltype(decomposer), GpuDirectInterleaver<T>, decltype(encoder),
                              AdaptiveLevelCompressor, decltype(collector), ConcatLevelFileWriter>
-                r{GpuMGARDOrthogonalDecomposer<T>(ctx), GpuDirectInterleaver<T>(ctx), encoder,
-                  AdaptiveLevelCompressor(64), collector, writer};
+                r{decomposer, GpuDirectInterleaver<T>(ctx), encoder, AdaptiveLevelCompressor(64), collector, writer};
             r.negabinary = !grouped;
             r.refactor(data.data(), dims, L, nb);
+        };
+        auto go = [&](auto encoder, auto collector) {
+            if (hier) go2(GpuMGARDHierarchicalDecomposer<T>(ctx), encoder, collector);
+            else go2(GpuMGARDOrthogonalDecomposer<T>(ctx), encoder, collector);
         };
         if (grouped) {
             if (squared) go(GpuGroupedBPEncoder<T, S>(ctx), GpuSquaredErrorCollector<T>(ctx));
@@ -56,7 +60,8 @@ static int run(const std::string &file, const std::vector<uint32_t> &dims, int L
         }
     } else {
         GpuRefactor<T, AdaptiveLevelCompressor, ConcatLevelFileWriter> r(
-            grouped ? PRODM_ENC_GROUPED : PRODM_ENC_NEGABINARY, AdaptiveLevelCompressor(64), writer, squared);
+            grouped ? PRODM_ENC_GROUPED : PRODM_ENC_NEGABINARY, AdaptiveLevelCompressor(64), writer, squared,
+            hier ? PRODM_DECOMP_HIERARCHICAL : PRODM_DECOMP_ORTHOGONAL);
         r.refactor(data.data(), dims, L, nb);
     }
     const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -78,7 +83,7 @@ int main(int argc, char **argv) {
     const int nd = std::atoi(argv[a++]);
     std::vector<uint32_t> dims;
     for (int i = 0; i < nd && a < argc; i++) dims.push_back((uint32_t)std::atoi(argv[a++]));
-    bool f64 = false, grouped = false, composed = false, squared = false;
+    bool f64 = false, grouped = false, composed = false, squared = false, hier = false;
     std::string dir = "refactored_data";
     for (; a < argc; a++) {
         const std::string o = argv[a];
@@ -86,6 +91,7 @@ int main(int argc, char **argv) {
         else if (o == "--grouped") grouped = true;
         else if (o == "--composed") composed = true;
         else if (o == "--squared") squared = true;
+        else if (o == "--hierarchical") hier = true;
         else if (o == "--out" && a + 1 < argc) dir = argv[++a];
         else fatal("unknown option " + o);
     }
@@ -96,6 +102,6 @@ int main(int argc, char **argv) {
     const int maxnb = f64 ? 64 : 32;
     if (nb > maxnb) nb = maxnb;
     mkdir(dir.c_str(), 0755);
-    return f64 ? run<double, uint64_t>(file, dims, L, nb, grouped, composed, squared, dir)
-               : run<float, uint32_t>(file, dims, L, nb, grouped, composed, squared, dir);
+    return f64 ? run<double, uint64_t>(file, dims, L, nb, grouped, composed, squared, hier, dir)
+               : run<float, uint32_t>(file, dims, L, nb, grouped, composed, squared, hier, dir);
 }

@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("PRODM_MDR_LIB") or os.path.join(HERE, "libprodm_mdr.so")
 
 F32, F64 = 0, 1
+ORTHOGONAL, HIERARCHICAL = 0, 1
 NEGABINARY, GROUPED = 0, 1
 
 _lib = None
@@ -31,6 +32,11 @@ _SIGS = {
     "prodm_recompose_adjoint_workspace_bytes": (sz, [vp, C.c_int, C.c_int]),
     "prodm_recompose_adjoint_f32": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp, vp, sz, vp]),
     "prodm_recompose_adjoint_f64": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp, vp, sz, vp]),
+    "prodm_decompose_hierarchical_f32": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp, vp, sz, vp]),
+    "prodm_decompose_hierarchical_f64": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp, vp, sz, vp]),
+    "prodm_recompose_hierarchical_f32": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp, vp, sz, vp]),
+    "prodm_recompose_hierarchical_f64": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp, vp, sz, vp]),
+    "prodm_mdr_plan_set_decomposer": (C.c_int, [vp, C.c_int]),
     "prodm_interleave_f32": (C.c_int, [vp, vp, C.c_int, vp, vp, vp, vp, vp]),
     "prodm_interleave_f64": (C.c_int, [vp, vp, C.c_int, vp, vp, vp, vp, vp]),
     "prodm_reposition_f32": (C.c_int, [vp, vp, C.c_int, vp, vp, vp, vp, vp]),

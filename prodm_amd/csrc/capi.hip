@@ -94,7 +94,7 @@ static int check_geom(const LevelGeom &g) {
 
 template <class T>
 static int decompose_impl(const T *in, T *out, const uint32_t *dims, int nd, int L, const uint32_t *strides, void *ws,
-                          size_t ws_bytes, hipStream_t s) {
+                          size_t ws_bytes, hipStream_t s, bool hier = false) {
     const LevelGeom g = make_geom(dims, nd, L);
     if (int rc = check_geom(g)) return rc;
     if (!in || !out || in == out) { set_error("decompose: need distinct device input and output"); return kErrArg; }
@@ -103,12 +103,12 @@ static int decompose_impl(const T *in, T *out, const uint32_t *dims, int nd, int
     CoefSink<T> sink;
     sink.full = out;
     sink.s[0] = st[0]; sink.s[1] = st[1]; sink.s[2] = 1;
-    return mgard_decompose<T>(in, st, g, sink, nullptr, ws, ws_bytes, s);
+    return mgard_decompose<T>(in, st, g, sink, nullptr, ws, ws_bytes, s, nullptr, false, hier);
 }
 
 template <class T>
 static int recompose_impl(const T *in, T *out, const uint32_t *dims, int nd, int L, const uint32_t *strides, void *ws,
-                          size_t ws_bytes, hipStream_t s) {
+                          size_t ws_bytes, hipStream_t s, bool hier = false) {
     const LevelGeom g = make_geom(dims, nd, L);
     if (int rc = check_geom(g)) return rc;
     if (!in || !out || in == out) { set_error("recompose: need distinct device input and output"); return kErrArg; }
@@ -117,7 +117,7 @@ static int recompose_impl(const T *in, T *out, const uint32_t *dims, int nd, int
     CoefSink<T> src;
     src.full = const_cast<T *>(in);
     src.s[0] = st[0]; src.s[1] = st[1]; src.s[2] = 1;
-    return mgard_recompose<T>(src, g, 0, out, st, ws, ws_bytes, s);
+    return mgard_recompose<T>(src, g, 0, out, st, ws, ws_bytes, s, nullptr, hier);
 }
 
 template <class T>
@@ -276,6 +276,7 @@ struct prodm_mdr_plan {
     size_t gbws_bytes = 0;
     uint64_t device_bytes = 0;
     prodm::StageTimer timer;
+    int hier = 0;  // PRODM_DECOMP_HIERARCHICAL
     // captured graphs: refactor of one input pointer, retrieve of one
     // (planes per level, output pointer) pair
     hipGraphExec_t g_ref = nullptr, g_ret = nullptr;
@@ -315,7 +316,7 @@ template <class T> static int plan_refactor(prodm_mdr_plan *p, const T *in, hipS
     const uint64_t st[3] = {(uint64_t)g.dims[1] * g.dims[2], g.dims[2], 1};
     StageTimer *tm = p->timer.on ? &p->timer : nullptr;
     if (tm) tm->stream = s;
-    rc = mgard_decompose<T>(in, st, g, sink, keys, p->ws, p->ws_bytes, s, tm, true);
+    rc = mgard_decompose<T>(in, st, g, sink, keys, p->ws, p->ws_bytes, s, tm, true, p->hier != 0);
     if (rc) return rc;
     for (int l = g.L; l >= 0; l--) {
         StageScope sc(tm, l == g.L ? kStEncTop : kStEncRest);
@@ -357,7 +358,7 @@ template <class T> static int plan_retrieve(prodm_mdr_plan *p, const uint8_t *k,
         if (rc) return rc;
     }
     const uint64_t st[3] = {(uint64_t)g.dims[1] * g.dims[2], g.dims[2], 1};
-    return mgard_recompose<T>(src, g, 1, out, st, p->ws, p->ws_bytes, s, tm);
+    return mgard_recompose<T>(src, g, 1, out, st, p->ws, p->ws_bytes, s, tm, p->hier != 0);
 }
 
 }  // namespace prodm
@@ -412,6 +413,35 @@ int prodm_recompose_f32(const float *d_in, float *d_out, const uint32_t *dims, i
 int prodm_recompose_f64(const double *d_in, double *d_out, const uint32_t *dims, int ndims, int target_level,
                         const uint32_t *strides, void *d_ws, size_t ws_bytes, prodm_stream_t stream) {
     return recompose_impl<double>(d_in, d_out, dims, ndims, target_level, strides, d_ws, ws_bytes, (hipStream_t)stream);
+}
+
+int prodm_decompose_hierarchical_f32(const float *d_in, float *d_out, const uint32_t *dims, int ndims, int target_level,
+                                     const uint32_t *strides, void *d_ws, size_t ws_bytes, prodm_stream_t stream) {
+    return decompose_impl<float>(d_in, d_out, dims, ndims, target_level, strides, d_ws, ws_bytes, (hipStream_t)stream, true);
+}
+int prodm_decompose_hierarchical_f64(const double *d_in, double *d_out, const uint32_t *dims, int ndims,
+                                     int target_level, const uint32_t *strides, void *d_ws, size_t ws_bytes,
+                                     prodm_stream_t stream) {
+    return decompose_impl<double>(d_in, d_out, dims, ndims, target_level, strides, d_ws, ws_bytes, (hipStream_t)stream, true);
+}
+int prodm_recompose_hierarchical_f32(const float *d_in, float *d_out, const uint32_t *dims, int ndims, int target_level,
+                                     const uint32_t *strides, void *d_ws, size_t ws_bytes, prodm_stream_t stream) {
+    return recompose_impl<float>(d_in, d_out, dims, ndims, target_level, strides, d_ws, ws_bytes, (hipStream_t)stream, true);
+}
+int prodm_recompose_hierarchical_f64(const double *d_in, double *d_out, const uint32_t *dims, int ndims,
+                                     int target_level, const uint32_t *strides, void *d_ws, size_t ws_bytes,
+                                     prodm_stream_t stream) {
+    return recompose_impl<double>(d_in, d_out, dims, ndims, target_level, strides, d_ws, ws_bytes, (hipStream_t)stream, true);
+}
+int prodm_mdr_plan_set_decomposer(prodm_mdr_plan *p, int kind) {
+    if (!p || (kind != PRODM_DECOMP_ORTHOGONAL && kind != PRODM_DECOMP_HIERARCHICAL)) {
+        set_error("plan_set_decomposer: bad arguments");
+        return kErrArg;
+    }
+    p->hier = kind == PRODM_DECOMP_HIERARCHICAL;
+    if (p->g_ref) { (void)hipGraphExecDestroy(p->g_ref); p->g_ref = nullptr; }
+    if (p->g_ret) { (void)hipGraphExecDestroy(p->g_ret); p->g_ret = nullptr; }
+    return kOk;
 }
 
 int prodm_interleave_f32(const float *d_data, const uint32_t *dims, int ndims, const uint32_t *dims_fine,

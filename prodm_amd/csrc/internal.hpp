@@ -95,6 +95,8 @@ template <class T> struct CoefSink {
     uint64_t s[3] = {0, 0, 1};   // strides of the full array
     T *level[33] = {nullptr};    // interleaved mode: level buffers (0..L)
 };
+// hier: hierarchical basis (MGARDHierarchicalDecomposer): the same level
+// steps without the L2 correction (no load vector, no line solves).
 // Thomas factor tables live in the workspace. tables_ready: they were
 // uploaded once by mgard_upload_tables (synchronous); otherwise each call
 // uploads them on the stream.
@@ -102,10 +104,10 @@ int mgard_upload_tables(const LevelGeom &g, size_t elem, void *ws, size_t ws_byt
 template <class T>
 int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g, const CoefSink<T> &sink,
                     KeyOf<T> *level_keys, void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm = nullptr,
-                    bool tables_ready = false);
+                    bool tables_ready = false, bool hier = false);
 template <class T>
 int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int tables_ready, T *out, const uint64_t *out_strides,
-                    void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm = nullptr);
+                    void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm = nullptr, bool hier = false);
 // Transpose of recompose (adjoint.hip): g_in = dJ/du on the level-L box
 // (strides in_strides), out = dJ/dcoefficients in the decompose layout.
 size_t adjoint_workspace_bytes(const LevelGeom &g);

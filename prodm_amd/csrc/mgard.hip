@@ -62,6 +62,7 @@ template <class T> struct LoadParams {
     uint32_t fp;           // row pitch of f
     KeyOf<T> *key;         // decompose: level max |W|
     int march;             // coarse planes per workgroup along dim 0
+    int hb;                // decompose: hierarchical basis (no load vector)
 };
 
 template <class T>
@@ -370,14 +371,18 @@ __global__ __launch_bounds__(kLoadThreads) void k_load(LoadParams<T> P) {
             __syncthreads();
             make_w(i0, E0, E1, O);
             __syncthreads();
-            stencil2();
-            __syncthreads();
-            column(i0);
+            if (!P.hb) {
+                stencil2();
+                __syncthreads();
+                column(i0);
+            }
             make_w(i0 + 1, E1, E1, E1);
             __syncthreads();
-            stencil2();
-            __syncthreads();
-            column(i0 + 1);
+            if (!P.hb) {
+                stencil2();
+                __syncthreads();
+                column(i0 + 1);
+            }
         };
         fetch(pa, i0lo);
         put(Ea, pa);
@@ -385,9 +390,11 @@ __global__ __launch_bounds__(kLoadThreads) void k_load(LoadParams<T> P) {
         __syncthreads();
         make_w(i0lo, Ea, Ea, Ea);
         __syncthreads();
-        stencil2();
-        __syncthreads();
-        column(i0lo);
+        if (!P.hb) {
+            stencil2();
+            __syncthreads();
+            column(i0lo);
+        }
         T *E0 = Ea, *E1 = Eb;
 #pragma unroll 1
         for (int i0 = i0lo + 1; i0 <= i0hi; i0 += 2) {
@@ -849,7 +856,8 @@ int mgard_upload_tables(const LevelGeom &g, size_t elem, void *ws, size_t ws_byt
 
 template <class T>
 int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g, const CoefSink<T> &sink,
-                    KeyOf<T> *level_keys, void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm, bool tables_ready) {
+                    KeyOf<T> *level_keys, void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm, bool tables_ready,
+                    bool hier) {
     const WsLayout w = ws_layout(g, sizeof(T));
     if (ws_bytes < w.total) { set_error("decompose: workspace too small"); return kErrArg; }
     char *base = static_cast<char *>(ws);
@@ -875,13 +883,14 @@ int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g,
         P.f = f;
         P.fp = f_pitch(P.c[2]);
         P.key = level_keys ? level_keys + l : nullptr;
+        P.hb = hier;
         int rc;
         {
             StageScope sc(tm, l == g.L ? kStDecLoadTop : kStDecLoadRest);
             rc = launch_load<T, kDecompose>(P, ilv, s);
         }
         if (rc) return rc;
-        {
+        if (!hier) {
             StageScope sc(tm, l == g.L ? kStDecSolveTop : kStDecSolveRest);
             rc = solve_all<T>(g, l, f, tab, F, kEpiAdd, uc, uc, s);
         }
@@ -912,13 +921,13 @@ int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g,
     return kOk;
 }
 template int mgard_decompose<float>(const float *, const uint64_t *, const LevelGeom &, const CoefSink<float> &,
-                                    uint32_t *, void *, size_t, hipStream_t, StageTimer *, bool);
+                                    uint32_t *, void *, size_t, hipStream_t, StageTimer *, bool, bool);
 template int mgard_decompose<double>(const double *, const uint64_t *, const LevelGeom &, const CoefSink<double> &,
-                                     unsigned long long *, void *, size_t, hipStream_t, StageTimer *, bool);
+                                     unsigned long long *, void *, size_t, hipStream_t, StageTimer *, bool, bool);
 
 template <class T>
 int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int tables_ready, T *out, const uint64_t *out_strides,
-                    void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm) {
+                    void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm, bool hier) {
     const WsLayout w = ws_layout(g, sizeof(T));
     if (ws_bytes < w.total) { set_error("recompose: workspace too small"); return kErrArg; }
     char *base = static_cast<char *>(ws);
@@ -945,16 +954,16 @@ int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int tables_ready
         LoadParams<T> P = load_params(g, l, src);
         P.f = f;
         P.fp = f_pitch(P.c[2]);
-        {
-            StageScope sc(tm, l == g.L ? kStRecLoadTop : kStRecLoadRest);
-            rc = launch_load<T, kRecompose>(P, ilv, s);
-        }
-        if (rc) return rc;
-        {
+        if (!hier) {
+            {
+                StageScope sc(tm, l == g.L ? kStRecLoadTop : kStRecLoadRest);
+                rc = launch_load<T, kRecompose>(P, ilv, s);
+            }
+            if (rc) return rc;
             StageScope sc(tm, l == g.L ? kStRecSolveTop : kStRecSolveRest);
             rc = solve_all<T>(g, l, f, tab, F, kEpiSub, cur, cur, s);  // u_c = u_c' - z, in place
+            if (rc) return rc;
         }
-        if (rc) return rc;
         const uint32_t *n = g.ld[l];
         T *dst;
         uint64_t d0, d1;
@@ -1006,8 +1015,8 @@ const double *mgard_adjoint_solution(const LevelGeom &g, int l, void *ws, uint32
 }
 
 template int mgard_recompose<float>(const CoefSink<float> &, const LevelGeom &, int, float *, const uint64_t *, void *,
-                                    size_t, hipStream_t, StageTimer *);
+                                    size_t, hipStream_t, StageTimer *, bool);
 template int mgard_recompose<double>(const CoefSink<double> &, const LevelGeom &, int, double *, const uint64_t *,
-                                     void *, size_t, hipStream_t, StageTimer *);
+                                     void *, size_t, hipStream_t, StageTimer *, bool);
 
 }  // namespace prodm

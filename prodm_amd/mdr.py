@@ -49,27 +49,37 @@ def _ws(nbytes):
 
 
 # ---- MGARD -------------------------------------------------------------------
-def decompose(x, dims, L, strides=None):
+def decompose(x, dims, L, strides=None, hierarchical=False):
     out = torch.empty_like(x)
     nb = lib().prodm_mgard_workspace_bytes(_u32(dims), len(dims), L, _TDT[x.dtype])
     if nb == 0:
         raise capi.ProdmError("unsupported geometry")
     ws = _ws(nb)
     st = _u32(strides) if strides is not None else None
-    check(_fn("prodm_decompose", x.dtype)(_ptr(x), _ptr(out), _u32(dims), len(dims), L, st, _ptr(ws), ws.numel(),
-                                          _stream()), "decompose")
+    name = "prodm_decompose_hierarchical" if hierarchical else "prodm_decompose"
+    check(_fn(name, x.dtype)(_ptr(x), _ptr(out), _u32(dims), len(dims), L, st, _ptr(ws), ws.numel(), _stream()),
+          "decompose")
     return out
 
 
-def recompose(c, dims, L, strides=None):
+def decompose_hierarchical(x, dims, L, strides=None):
+    return decompose(x, dims, L, strides, hierarchical=True)
+
+
+def recompose_hierarchical(c, dims, L, strides=None):
+    return recompose(c, dims, L, strides, hierarchical=True)
+
+
+def recompose(c, dims, L, strides=None, hierarchical=False):
     out = torch.empty_like(c)
     nb = lib().prodm_mgard_workspace_bytes(_u32(dims), len(dims), L, _TDT[c.dtype])
     if nb == 0:
         raise capi.ProdmError("unsupported geometry")
     ws = _ws(nb)
     st = _u32(strides) if strides is not None else None
-    check(_fn("prodm_recompose", c.dtype)(_ptr(c), _ptr(out), _u32(dims), len(dims), L, st, _ptr(ws), ws.numel(),
-                                          _stream()), "recompose")
+    name = "prodm_recompose_hierarchical" if hierarchical else "prodm_recompose"
+    check(_fn(name, c.dtype)(_ptr(c), _ptr(out), _u32(dims), len(dims), L, st, _ptr(ws), ws.numel(), _stream()),
+          "recompose")
     return out
 
 
@@ -218,11 +228,13 @@ def collect_squared(x, nb, maxerr):
 class Plan:
     """Device-resident refactor/retrieve of one field (prodm_mdr_plan)."""
 
-    def __init__(self, dims, L, nb, dtype=torch.float32, encoder=capi.NEGABINARY):
+    def __init__(self, dims, L, nb, dtype=torch.float32, encoder=capi.NEGABINARY, decomposer=capi.ORTHOGONAL):
         self.dims, self.L, self.nb, self.dtype, self.encoder = list(dims), L, nb, dtype, encoder
         h = C.c_void_p()
         check(lib().prodm_mdr_plan_create(C.byref(h), _u32(dims), len(dims), L, nb, _TDT[dtype], encoder), "plan")
         self.h = h
+        if decomposer != capi.ORTHOGONAL:
+            check(lib().prodm_mdr_plan_set_decomposer(self.h, decomposer), "plan decomposer")
         self.level_dims, self.level_elements, _ = level_geometry(dims, L)
 
     @property

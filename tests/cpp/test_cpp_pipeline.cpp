@@ -168,6 +168,49 @@ static void squared_path(const std::string &root, const std::vector<float> &u, c
     }
 }
 
+// ProDM's own driver pairing (test/test_mdr_refactor.cpp:64-70,
+// test_mdr_reconstructor.cpp:70-83): hierarchical decomposer, negabinary,
+// MaxErrorEstimatorHB with the sign-excluding greedy planner.
+static void hierarchical_path(const std::string &root, const std::vector<float> &u, const std::vector<uint32_t> &dims,
+                              int L, int nb) {
+    const std::string a = root + "/composed_hb", b = root + "/gpu_hb";
+    mkdir(a.c_str(), 0755);
+    mkdir(b.c_str(), 0755);
+    Context ctx = make_context();
+    {
+        ComposedRefactor<float, GpuMGARDHierarchicalDecomposer<float>, GpuDirectInterleaver<float>,
+                         GpuNegaBinaryBPEncoder<float, uint32_t>, AdaptiveLevelCompressor, MaxErrorCollector<float>,
+                         ConcatLevelFileWriter>
+            r{GpuMGARDHierarchicalDecomposer<float>(ctx), GpuDirectInterleaver<float>(ctx),
+              GpuNegaBinaryBPEncoder<float, uint32_t>(ctx), AdaptiveLevelCompressor(64), MaxErrorCollector<float>(),
+              ConcatLevelFileWriter(a + "/metadata.bin", level_files(a, L))};
+        r.negabinary = true;
+        r.refactor(u.data(), dims, L, nb);
+    }
+    {
+        GpuRefactor<float, AdaptiveLevelCompressor, ConcatLevelFileWriter> r(
+            PRODM_ENC_NEGABINARY, AdaptiveLevelCompressor(64), ConcatLevelFileWriter(b + "/metadata.bin", level_files(b, L)),
+            false, PRODM_DECOMP_HIERARCHICAL);
+        r.refactor(u.data(), dims, L, nb);
+    }
+    EXPECT(slurp(a + "/metadata.bin") == slurp(b + "/metadata.bin"), "hierarchical metadata differs");
+    for (int l = 0; l <= L; l++)
+        EXPECT(slurp(level_files(a, L)[l]) == slurp(level_files(b, L)[l]), "hierarchical level %d file differs", l);
+    using Est = MaxErrorEstimatorHB<float>;
+    using Interp = SignExcludeGreedyBasedSizeInterpreter<Est>;
+    GpuReconstructor<float, AdaptiveLevelCompressor, Interp, Est, ConcatLevelFileRetriever> rg(
+        AdaptiveLevelCompressor(64), Interp(Est()), ConcatLevelFileRetriever(b + "/metadata.bin", level_files(b, L)),
+        PRODM_DECOMP_HIERARCHICAL);
+    rg.load_metadata();
+    const double R = range_of(u);
+    for (double rel : {1e-1, 1e-3, 1e-5}) {
+        const float *g = rg.progressive_reconstruct(rel * R);
+        const double e = linf(u, g);
+        std::printf("hierarchical tol %.1e: gpu err %.3e, retrieved %zu bytes\n", rel, e / R, rg.get_retrieved_size());
+        EXPECT(e <= rel * R, "hierarchical error %.3e above %.3e", e, rel * R);
+    }
+}
+
 template <class Encoder>
 static void component_checks(const std::vector<float> &u, int nb, Encoder enc) {
     const int n = (int)u.size() / 7 + 5;
@@ -204,6 +247,7 @@ int main(int argc, char **argv) {
     roundtrip_files(root, u, dims, L, nb, true, GpuNegaBinaryBPEncoder<float, uint32_t>(ctx));
     roundtrip_files(root, u, dims, L, nb, false, GpuGroupedBPEncoder<float, uint32_t>(ctx));
     squared_path(root, u, dims, L, nb);
+    hierarchical_path(root, u, dims, L, nb);
     component_checks(u, nb, GpuNegaBinaryBPEncoder<float, uint32_t>(ctx));
     component_checks(u, nb, GpuGroupedBPEncoder<float, uint32_t>(ctx));
     {  // interleave -> reposition is the identity on every level

@@ -247,21 +247,21 @@ def collect_squared(x, nb, maxerr, mode="exact"):
 
 
 # ---- MGARD ----------------------------------------------------------------------------
-def decompose(data, dims, L, strides=None):
+def decompose(data, dims, L, strides=None, hierarchical=False):
     t = _tp(data.dtype)
     out = np.ascontiguousarray(data).copy()
     st = None if strides is None else _arr_u32(strides).ctypes.data_as(C.c_void_p)
-    rc = getattr(oracle(), "orc_decompose_" + t)(out.ctypes.data_as(C.c_void_p), _arr_u32(dims).ctypes.data_as(C.c_void_p),
+    rc = getattr(oracle(), ("orc_decompose_hb_" if hierarchical else "orc_decompose_") + t)(out.ctypes.data_as(C.c_void_p), _arr_u32(dims).ctypes.data_as(C.c_void_p),
                                                   len(dims), L, st)
     assert rc == 0, rc
     return out
 
 
-def recompose(data, dims, L, strides=None):
+def recompose(data, dims, L, strides=None, hierarchical=False):
     t = _tp(data.dtype)
     out = np.ascontiguousarray(data).copy()
     st = None if strides is None else _arr_u32(strides).ctypes.data_as(C.c_void_p)
-    rc = getattr(oracle(), "orc_recompose_" + t)(out.ctypes.data_as(C.c_void_p), _arr_u32(dims).ctypes.data_as(C.c_void_p),
+    rc = getattr(oracle(), ("orc_recompose_hb_" if hierarchical else "orc_recompose_") + t)(out.ctypes.data_as(C.c_void_p), _arr_u32(dims).ctypes.data_as(C.c_void_p),
                                                   len(dims), L, st)
     assert rc == 0, rc
     return out

@@ -30,7 +30,7 @@ def test_cpp_pipeline(gpu, tmp_path):
     assert "PASSED" in r.stdout
 
 
-@pytest.mark.parametrize("mode", [[], ["--grouped"], ["--composed"], ["--squared"]])
+@pytest.mark.parametrize("mode", [[], ["--grouped"], ["--composed"], ["--squared"], ["--hierarchical"]])
 def test_apps_refactor_reconstruct(gpu, tmp_path, mode):
     dims = [65, 65, 33]
     u = bumps_field(dims, seed=5)
@@ -41,15 +41,15 @@ def test_apps_refactor_reconstruct(gpu, tmp_path, mode):
     assert r.returncode == 0
     R = float(u.max() - u.min())
     tols = [1e-1 * R, 1e-3 * R, 1e-5 * R]
-    extra = ["--l2"] if mode == ["--squared"] else []
-    if extra:
+    extra = ["--l2"] if mode == ["--squared"] else (["--hierarchical"] if mode == ["--hierarchical"] else [])
+    if extra == ["--l2"]:
         tols = [(t * float(np.linalg.norm(u))) ** 2 for t in (1e-2, 1e-4)]
     r = run([os.path.join(BIN, "prodm_reconstruct"), str(f), str(len(tols)), *map(repr, tols), "--in", str(out),
              *extra])
     assert r.returncode == 0
     lines = [l for l in r.stdout.splitlines() if l.startswith("tolerance")]
     assert len(lines) == len(tols)
-    if not extra:
+    if extra != ["--l2"]:
         for t, line in zip(tols, lines):
             err = float(line.split("max error ")[1].split()[0])
             assert err <= t, line

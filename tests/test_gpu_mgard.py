@@ -79,3 +79,40 @@ def test_bad_geometry_rejected(gpu):
     x = torch.zeros(32 * 32, dtype=torch.float32, device=gpu)
     with pytest.raises(capi.ProdmError):
         mdr.decompose(x, [32, 32], 2)
+
+
+
+@pytest.mark.parametrize("dims,L,dt", [([65, 33, 17], 3, np.float32), ([129, 65], 5, np.float64), ([257], 7, np.float32)])
+def test_hierarchical_matches_oracle(gpu, dims, L, dt):
+    """MGARDHierarchicalDecomposer (no L2 correction) on the GPU against the
+    oracle, both directions, plus the plan path's parity in that mode."""
+    from prodm_amd import mdr
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal(int(np.prod(dims))).astype(dt)
+    x = torch.from_numpy(u).to(gpu)
+    d = mdr.decompose(x, dims, L, hierarchical=True).cpu().numpy()
+    ref = ob.decompose(u, dims, L, hierarchical=True)
+    tol = 1e-6 if dt == np.float32 else 1e-14
+    assert np.max(np.abs(d.astype(np.float64) - ref)) <= tol * np.max(np.abs(ref))
+    r = mdr.recompose(torch.from_numpy(ref.astype(dt)).to(gpu), dims, L, hierarchical=True).cpu().numpy()
+    want = ob.recompose(ref.astype(dt), dims, L, hierarchical=True)
+    assert np.max(np.abs(r.astype(np.float64) - want)) <= tol * np.max(np.abs(want))
+
+
+def test_plan_hierarchical_round_trip(gpu):
+    from prodm_amd import capi, mdr
+    dims, L, nb = [65, 65, 65], 4, 32
+    u = np.random.default_rng(4).standard_normal(65 ** 3).astype(np.float32)
+    plan = mdr.Plan(dims, L, nb, torch.float32, capi.NEGABINARY, decomposer=capi.HIERARCHICAL)
+    x = torch.from_numpy(u).to(gpu)
+    plan.refactor(x)
+    dec = ob.decompose(u, dims, L, hierarchical=True)
+    ld = ob.level_dims(dims, L)
+    for l in range(L + 1):
+        coarse = ld[l - 1] if l else np.zeros(3, np.uint32)
+        want = ob.interleave(dec, dims, ld[l], coarse)
+        got = plan.coefficients(l).cpu().numpy()
+        assert np.max(np.abs(got.astype(np.float64) - want)) <= 1e-6 * np.max(np.abs(dec))
+    out = plan.retrieve([nb] * (L + 1)).cpu().numpy()
+    assert np.max(np.abs(out.astype(np.float64) - u)) <= 1e-5 * np.max(np.abs(u))
+    plan.close()

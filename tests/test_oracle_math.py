@@ -346,3 +346,34 @@ def test_squared_collector_is_exactly_rounded(nb):
     want = [float(sum((Fraction(float(t[k])) for t in terms), Fraction(0))) for k in range(nb + 1)]
     got = ob.collect_squared(x, nb, m, mode="exact")
     np.testing.assert_array_equal(got, np.array(want))
+
+
+# ------------------------------------------------------- hierarchical basis ---
+
+
+@pytest.mark.parametrize("dims,L", [([9, 17, 5], 2), ([33, 17], 3), ([65], 5)])
+def test_hierarchical_keeps_nodal_values(dims, L):
+    """The hierarchical basis subtracts the interpolant only: the level-0 grid
+    holds the field's own values at its nodes, every coefficient is u - I u_c
+    with u_c the original nodal values, and recompose inverts it."""
+    rng = np.random.default_rng(8)
+    u = rng.standard_normal(int(np.prod(dims)))
+    out = ob.decompose(u, dims, L, hierarchical=True).reshape(dims)
+    ld = ob.level_dims(dims, L)
+    step = [(n - 1) // (c - 1) if c > 1 else 1 for n, c in zip(dims, ld[0])]
+    np.testing.assert_array_equal(out[tuple(slice(0, int(n)) for n in ld[0])],
+                                  u.reshape(dims)[tuple(slice(0, None, st) for st in step)])
+    one = ob.decompose(u, dims, 1, hierarchical=True).reshape(dims)
+    P = kron_all([prolong_1d(n) for n in dims])
+    U = u.reshape(dims)
+    coarse_idx = [np.arange(0, n, 2) if n > 1 else np.arange(1) for n in dims]
+    W = (u - P @ U[np.ix_(*coarse_idx)].ravel()).reshape(dims)
+    m = [(n - 1) // 2 for n in dims]
+    # odd indices sit after the coarse nodes along each dim
+    idx = tuple(np.concatenate([np.arange(0, n, 2), np.arange(1, n, 2)]) if n > 1 else np.arange(1) for n in dims)
+    Wr = W[np.ix_(*idx)]
+    mask = np.ones(dims, bool)
+    mask[tuple(slice(0, mm + 1) if n > 1 else slice(0, 1) for mm, n in zip(m, dims))] = False
+    np.testing.assert_allclose(one[mask], Wr[mask], rtol=0, atol=1e-13)
+    back = ob.recompose(ob.decompose(u, dims, L, hierarchical=True), dims, L, hierarchical=True)
+    np.testing.assert_allclose(back, u, rtol=0, atol=1e-14 * np.max(np.abs(u)))
