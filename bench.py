@@ -53,6 +53,138 @@ def make_field_gpu(n, seed=0, nbumps=16, noise=1e-4):
     return u.to(torch.float32).contiguous().view(-1)
 
 
+def batch_shapes(nfields=96, seed=3):
+    """Shapes of the batched workload: per field ndim in {2, 3}, each dim
+    2^k + 1 with k ~ U{5..9}, redrawn while the field has more than 2^25
+    points; L = min(6, k_min - 1), at least 4. numpy PCG64 with `seed`."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nfields):
+        nd = int(rng.integers(2, 4))
+        while True:
+            ks = [int(k) for k in rng.integers(5, 10, nd)]
+            dims = [2 ** k + 1 for k in ks]
+            if int(np.prod(dims)) <= 2 ** 25:
+                break
+        out.append((dims, max(4, min(6, min(ks) - 1))))
+    return out
+
+
+def make_batch_field_gpu(dims, seed, nbumps=8, noise=1e-3):
+    """The 129^3 workload's generator on any 2-D / 3-D box: sin(2pi 3x)
+    cos(2pi 5y) [sin(2pi 2z)] + 8 Gaussian bumps (A, w, c drawn by numpy
+    PCG64 in that order) + N(0, noise) from torch's generator seeded with
+    `seed`; fp64 on the GPU, returned as dense float32."""
+    import torch
+    rng = np.random.default_rng(seed)
+    nd = len(dims)
+    A = rng.uniform(0.5, 2.0, nbumps)
+    w = rng.uniform(0.02, 0.1, nbumps)
+    c = rng.uniform(0.0, 1.0, (nbumps, nd))
+    dev = torch.device("cuda")
+    ax = []
+    for d, n in enumerate(dims):
+        shape = [1] * nd
+        shape[d] = n
+        ax.append(torch.linspace(0.0, 1.0, n, dtype=torch.float64, device=dev).view(shape))
+    u = torch.sin(2 * math.pi * 3 * ax[0]) * torch.cos(2 * math.pi * 5 * ax[1])
+    if nd > 2:
+        u = u * torch.sin(2 * math.pi * 2 * ax[2])
+    for b in range(nbumps):
+        g = None
+        for d in range(nd):
+            t = torch.exp(-((ax[d] - c[b, d]) ** 2) / (2 * w[b] ** 2))
+            g = t if g is None else g * t
+        u = u + A[b] * g
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    u = u + noise * torch.randn(tuple(dims), dtype=torch.float64, device=dev, generator=gen)
+    return u.to(torch.float32).contiguous().view(-1)
+
+
+def batched_report(steps, warmup, nb=32):
+    """The 96-field batched workload: per-field plan refactors (eager and
+    per-plan graphs) against the batched refactor (default shapes, autotuned
+    shapes, graph replay). Wall clock over `steps` runs on one non-default
+    stream, inputs resident; finest-level planes and all level maxima of
+    every field are checked bit-identical between the plain and batched runs."""
+    import torch
+    from prodm_amd import capi, mdr
+    from prodm_amd.batched import Batch
+    shapes = batch_shapes()
+    xs = [make_batch_field_gpu(d, 1000 + i) for i, (d, L) in enumerate(shapes)]
+    pts = sum(int(np.prod(d)) for d, L in shapes)
+    in_bytes = 4 * pts
+    plans = [mdr.Plan(d, L, nb, torch.float32, capi.NEGABINARY) for d, L in shapes]
+    b = Batch([(x, d, L) for x, (d, L) in zip(xs, shapes)], nb)
+    s = torch.cuda.Stream()
+
+    def timed(fn):
+        for _ in range(max(1, warmup)):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    def plain():
+        for p, x in zip(plans, xs):
+            p.refactor(x)
+
+    def plain_graph():
+        for p, x in zip(plans, xs):
+            p.refactor_graph(x)
+
+    with torch.cuda.stream(s):
+        t_plain = timed(plain)
+        t_plain_graph = timed(plain_graph)
+        t_default = timed(b.refactor)
+        t0 = time.perf_counter()
+        b.autotune(None, 3)
+        t_tune = time.perf_counter() - t0
+        t_tuned = timed(b.refactor)
+        t_graph = timed(b.refactor_graph)
+        plain()
+        b.refactor_graph()
+    torch.cuda.synchronize()
+    same = True
+    for f, (d, L) in enumerate(shapes):
+        bp = b.plan(f)
+        for l in range(L + 1):
+            same &= bp.level_info(l) == plans[f].level_info(l)
+        a, na = bp.plane(L, 0)
+        c, nc = plans[f].plane(L, 0)
+        stride = bp.plane(L, 1)[0] - a
+        ta = torch.empty(nb * stride, dtype=torch.uint8, device="cuda")
+        tc = torch.empty_like(ta)
+        mdr._d2d(ta, a, nb * stride)
+        mdr._d2d(tc, c, nb * stride)
+        same &= na == nc and torch.equal(ta.view(nb, stride)[:, :na], tc.view(nb, stride)[:, :nc])
+    launches = b.describe().strip().splitlines()
+    best = min(t_tuned, t_graph)
+    res = {
+        "workload": f"{len(shapes)} float32 fields ({sum(len(d) == 2 for d, _ in shapes)} 2-D, "
+                    f"{sum(len(d) == 3 for d, _ in shapes)} 3-D), dims 2^k+1 with k ~ U{{5..9}} (<= 2^25 points "
+                    f"each), L = min(6, k_min - 1) >= 4, {nb} bitplanes, NegaBinary; config-0 generator, "
+                    f"seeds 1000+f; refactor only",
+        "fields": len(shapes), "points": pts, "input_bytes": in_bytes,
+        "plain_ms": round(t_plain, 4), "plain_graph_ms": round(t_plain_graph, 4),
+        "batched_default_ms": round(t_default, 4), "batched_tuned_ms": round(t_tuned, 4),
+        "batched_graph_ms": round(t_graph, 4), "autotune_s": round(t_tune, 2),
+        "plain_GBps": round(in_bytes / (t_plain * 1e6), 2), "batched_GBps": round(in_bytes / (best * 1e6), 2),
+        "speedup_vs_plain": round(t_plain / best, 3), "speedup_vs_plain_graph": round(t_plain_graph / best, 3),
+        "grouped_launches": len(launches), "bit_identical": bool(same),
+    }
+    log(f"[bench] batched: plain {t_plain:.3f} ms (graphs {t_plain_graph:.3f}), batched default {t_default:.3f}, "
+        f"tuned {t_tuned:.3f}, graph {t_graph:.3f} ms; {len(launches)} grouped launches; identical={same}")
+    for p in plans:
+        p.close()
+    b.close()
+    return res
+
+
 def plan_planes(level_bounds, level_plane_bytes, nb, tol):
     """Planes per level for an L-infinity tolerance: the host C++ greedy
     planner (GreedyBasedSizeInterpreter.hpp:23-70) over the MaxErrorCollector
@@ -141,6 +273,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph", action="store_true", help="time HIP graph replays instead of eager launches")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 96-field batched refactor report")
+    ap.add_argument("--batched-only", action="store_true", help="run only the batched report (one JSON line)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only a plain copy kernel of known traffic (for counter scaling) and exit")
     args = ap.parse_args()
@@ -154,6 +288,10 @@ def main():
 
     import torch
     from prodm_amd import capi, mdr
+
+    if args.batched_only:
+        print(json.dumps(batched_report(args.steps, args.warmup)), flush=True)
+        return
 
     n, L, nb = args.size, args.levels, args.bitplanes
     dims = [n, n, n]
@@ -343,8 +481,10 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(res), flush=True)
     plan.close()
+    if not args.no_batched:
+        res["batched"] = batched_report(args.steps, args.warmup)
+    print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":

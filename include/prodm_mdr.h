@@ -251,6 +251,45 @@ int prodm_interpret_retrieve_size(int planner, int estimator, int dtype, int ndi
  * arithmetic; 0 for ndims outside 1..3.                                    */
 double prodm_max_estimator_ob_constant(int ndims, int dtype);
 
+/* ---- batched refactor (no counterpart in ProDM, which refactors one field
+ *      per ComposedRefactor::refactor call, ComposedRefactor.hpp:78-141):
+ *      many fields of uneven shapes refactored together with one grouped
+ *      launch per kernel and level step. A batch owns one plan per field:
+ *      prodm_mdr_batch_plan returns it, so prodm_mdr_plane,
+ *      prodm_mdr_level_info, prodm_mdr_plane_stream and prodm_mdr_retrieve
+ *      read and retrieve a batch field. Each field's planes and level maxima
+ *      are bit-identical to prodm_mdr_refactor of that field alone.       */
+typedef struct prodm_field_desc {
+    const void *d_in;       /* dense row-major device field, dims[0] slowest */
+    uint32_t dims[3];       /* first ndims used */
+    int ndims;
+    int target_level;       /* >= 1 */
+} prodm_field_desc;
+typedef struct prodm_mdr_batch prodm_mdr_batch;
+int prodm_mdr_batch_create(prodm_mdr_batch **out, const prodm_field_desc *fields, int nfields, int num_bitplanes,
+                           int dtype, int encoder);
+int prodm_mdr_batch_destroy(prodm_mdr_batch *batch);
+int prodm_mdr_batch_fields(const prodm_mdr_batch *batch);
+uint64_t prodm_mdr_batch_device_bytes(const prodm_mdr_batch *batch);
+/* the plan holding field f's results (owned by the batch)                 */
+prodm_mdr_plan *prodm_mdr_batch_plan(prodm_mdr_batch *batch, int field);
+int prodm_mdr_batch_set_input(prodm_mdr_batch *batch, int field, const void *d_in);
+int prodm_mdr_batch_set_decomposer(prodm_mdr_batch *batch, int kind);
+/* decompose + interleave + level maxima + encode of every field           */
+int prodm_mdr_batched_refactor(prodm_mdr_batch *batch, prodm_stream_t stream);
+/* the same sequence captured once into a HIP graph and replayed (stream
+ * must not be the default stream); inputs or shapes changed since the
+ * capture trigger a new capture                                            */
+int prodm_mdr_batched_refactor_graph(prodm_mdr_batch *batch, prodm_stream_t stream);
+/* Chooses each grouped launch's workgroup count and tile / block shape by
+ * timing `reps` runs of every candidate (launch by launch); with cache_path
+ * the choice is read from that text file when its signature (fields,
+ * dtype, encoder, device) matches, and written there otherwise. Results
+ * never depend on the shapes. Synchronizes.                                */
+int prodm_mdr_batch_autotune(prodm_mdr_batch *batch, const char *cache_path, int reps, prodm_stream_t stream);
+/* one text line per grouped launch: kind, entries, workgroups, shapes     */
+int prodm_mdr_batch_describe(prodm_mdr_batch *batch, char *buf, uint64_t capacity, uint64_t *length);
+
 #define PRODM_NUM_STAGES 15
 int prodm_mdr_profile(prodm_mdr_plan *plan, int enable);
 int prodm_mdr_stage_times(prodm_mdr_plan *plan, double *ms, uint32_t *counts, int nstages);

@@ -78,7 +78,23 @@ _SIGS = {
     "prodm_interpret_retrieve_size": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
                                                 vp, vp, vp, C.c_double, vp, vp]),
     "prodm_max_estimator_ob_constant": (C.c_double, [C.c_int, C.c_int]),
+    "prodm_mdr_batch_create": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "prodm_mdr_batch_destroy": (C.c_int, [vp]),
+    "prodm_mdr_batch_fields": (C.c_int, [vp]),
+    "prodm_mdr_batch_device_bytes": (u64, [vp]),
+    "prodm_mdr_batch_plan": (vp, [vp, C.c_int]),
+    "prodm_mdr_batch_set_input": (C.c_int, [vp, C.c_int, vp]),
+    "prodm_mdr_batch_set_decomposer": (C.c_int, [vp, C.c_int]),
+    "prodm_mdr_batched_refactor": (C.c_int, [vp, vp]),
+    "prodm_mdr_batched_refactor_graph": (C.c_int, [vp, vp]),
+    "prodm_mdr_batch_autotune": (C.c_int, [vp, C.c_char_p, C.c_int, vp]),
+    "prodm_mdr_batch_describe": (C.c_int, [vp, vp, u64, vp]),
 }
+
+
+class FieldDesc(C.Structure):
+    """prodm_field_desc: one field of a batched refactor."""
+    _fields_ = [("d_in", C.c_void_p), ("dims", C.c_uint32 * 3), ("ndims", C.c_int), ("target_level", C.c_int)]
 
 
 STAGES = ["decompose_load_top", "decompose_load_rest", "decompose_solve_top", "decompose_solve_rest",

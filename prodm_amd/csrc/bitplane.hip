@@ -8,6 +8,8 @@
 // holds bit k of every element, LSB = first element of the block
 // (GroupedBPEncoder.hpp:345, NegaBinaryBPEncoder.hpp:246). Plane stores and
 // loads are then one coalesced word per lane per plane.
+#include <algorithm>
+#include <cstring>
 #include <type_traits>
 
 #include "internal.hpp"
@@ -149,8 +151,8 @@ template <> struct BlkStage<double> {
 };
 
 // coalesced load of the workgroup's TPB*B elements into LDS rows (0 past n)
-template <class T>
-__device__ inline void stage_in(const T *__restrict__ x, uint64_t n, uint64_t e0, T *sx) {
+template <class T, class XP = const T *>
+__device__ inline void stage_in(XP __restrict__ x, uint64_t n, uint64_t e0, T *sx) {
     constexpr int TPB = BlkStage<T>::kTPB, B = BlkStage<T>::kB, PI = BlkStage<T>::kPitch;
     // all B loads in flight before the first LDS store
     T v[B];
@@ -188,15 +190,18 @@ template <class T> __device__ inline typename Traits<T>::Fixed to_negabinary(T v
     return ((F)q + (F)Traits<T>::kNbMask) ^ (F)Traits<T>::kNbMask;
 }
 
+// workgroup bid encodes bitplane blocks [bid TPB, (bid + 1) TPB)
 template <class T>
-__global__ __launch_bounds__(BlkStage<T>::kTPB) void k_nb_encode(const T *__restrict__ x, uint32_t n, int32_t exp,
-                                                              const KeyOf<T> *key, int nb,
-                                                              typename Traits<T>::Stream *__restrict__ planes,
-                                                              uint64_t stride) {
+__device__ __forceinline__ void nb_encode_blocks(const T *__restrict__ x, uint32_t n, int32_t exp, const KeyOf<T> *key,
+                                                 int nb, typename Traits<T>::Stream *__restrict__ planes,
+                                                 uint64_t stride, uint32_t bid) {
     constexpr int TPB = BlkStage<T>::kTPB, B = BlkStage<T>::kB, PI = BlkStage<T>::kPitch;
     __shared__ T sx[TPB * PI];
-    const uint64_t b0 = (uint64_t)blockIdx.x * TPB;
-    stage_in<T>(x, n, b0 * B, sx);
+    const uint64_t b0 = (uint64_t)bid * TPB;
+    // global-space pointers: the grouped form reads them from memory
+    using GS = PRODM_GLOBAL typename Traits<T>::Stream;
+    GS *gplanes = (GS *)planes;
+    stage_in<T, const PRODM_GLOBAL T *>((const PRODM_GLOBAL T *)x, n, b0 * B, sx);
     __syncthreads();
     const int shift = nb - (level_exp<T>(exp, key) + 2);  // exp += 2 (:21)
     const uint64_t b = b0 + threadIdx.x;
@@ -211,7 +216,7 @@ __global__ __launch_bounds__(BlkStage<T>::kTPB) void k_nb_encode(const T *__rest
 #pragma unroll
             for (int k = 0; k < 32; k++) {
                 const int p = nb - 1 - k;  // plane p holds bit nb - 1 - p
-                if (p >= 0) planes[(uint64_t)p * stride + b] = q[k];
+                if (p >= 0) gplanes[(uint64_t)p * stride + b] = q[k];
             }
         }
     } else {
@@ -227,10 +232,131 @@ __global__ __launch_bounds__(BlkStage<T>::kTPB) void k_nb_encode(const T *__rest
             static_for<0, 64>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
                 const int p = nb - 1 - k;
-                if (p >= 0) planes[(uint64_t)p * stride + b] = word64_get<k>(lo, hi);
+                if (p >= 0) gplanes[(uint64_t)p * stride + b] = word64_get<k>(lo, hi);
             });
         }
     }
+}
+
+template <class T>
+__global__ __launch_bounds__(BlkStage<T>::kTPB) void k_nb_encode(const T *__restrict__ x, uint32_t n, int32_t exp,
+                                                              const KeyOf<T> *key, int nb,
+                                                              typename Traits<T>::Stream *__restrict__ planes,
+                                                              uint64_t stride) {
+    nb_encode_blocks<T>(x, n, exp, key, nb, planes, stride, blockIdx.x);
+}
+
+// grouped form: one entry per level buffer of the batch (exponent from the
+// level's max key, as the plan's refactor encodes)
+template <class T> struct NbEntry {
+    const T *x;
+    const KeyOf<T> *key;
+    typename Traits<T>::Stream *planes;
+    uint64_t stride;
+    uint32_t n, pad;
+};
+template <class T>
+__global__ __launch_bounds__(BlkStage<T>::kTPB) void k_nb_encode_grouped(const NbEntry<T> *__restrict__ ps,
+                                                                      const uint32_t *__restrict__ first,
+                                                                      uint32_t nent, uint32_t total, int nb) {
+    for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const uint32_t e = group_entry(first, nent, vb);
+        const NbEntry<T> E = uniform_load(ps + e);
+        const uint32_t f0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)first[e]);
+        nb_encode_blocks<T>(E.x, E.n, 0, E.key, nb, E.planes, E.stride, vb - f0);
+        __syncthreads();  // the staging rows are reused by the next workgroup's blocks
+    }
+}
+
+template <class T>
+int nb_batched_launch(const std::vector<EncodeRef> &levels, int nb, const GroupShape &shape, GroupLaunch &L) {
+    constexpr int B = Traits<T>::kBlock, TPB = BlkStage<T>::kTPB;
+    if (nb < 1 || nb > B) { set_error("batched negabinary encode: bad num_bitplanes"); return kErrArg; }
+    L = GroupLaunch{};
+    L.kind = kGrpNbEncode;
+    L.elem = sizeof(T);
+    L.nb = nb;
+    L.block = TPB;
+    L.shape = shape;
+    std::vector<std::pair<uint32_t, NbEntry<T>>> v;
+    for (const EncodeRef &r : levels) {
+        if (!r.n) continue;
+        const uint64_t nw = ((uint64_t)r.n + B - 1) / B;
+        if (r.stride < nw || !r.x || !r.planes || !r.key) { set_error("batched negabinary encode: bad level"); return kErrArg; }
+        NbEntry<T> E{};
+        E.x = static_cast<const T *>(r.x);
+        E.key = static_cast<const KeyOf<T> *>(r.key);
+        E.planes = static_cast<typename Traits<T>::Stream *>(r.planes);
+        E.stride = r.stride;
+        E.n = r.n;
+        v.push_back({div_up(nw, TPB), E});
+    }
+    std::stable_sort(v.begin(), v.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+    L.first.push_back(0);
+    for (auto &x : v) {
+        const size_t o = L.params.size();
+        L.params.resize(o + sizeof(NbEntry<T>));
+        std::memcpy(L.params.data() + o, &x.second, sizeof(NbEntry<T>));
+        L.first.push_back(L.first.back() + x.first);
+        L.nent++;
+    }
+    L.total = L.first.back();
+    return kOk;
+}
+template int nb_batched_launch<float>(const std::vector<EncodeRef> &, int, const GroupShape &, GroupLaunch &);
+template int nb_batched_launch<double>(const std::vector<EncodeRef> &, int, const GroupShape &, GroupLaunch &);
+
+struct ZeroEntry {
+    uint32_t *p;
+    uint32_t words, pad;
+};
+__global__ __launch_bounds__(kThreads) void k_zero_grouped(const ZeroEntry *__restrict__ ps,
+                                                          const uint32_t *__restrict__ first, uint32_t nent,
+                                                          uint32_t total) {
+    for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const uint32_t e = group_entry(first, nent, vb);
+        const ZeroEntry E = uniform_load(ps + e);
+        const uint32_t f0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)first[e]);
+        const uint32_t i = (vb - f0) * kThreads + threadIdx.x;
+        if (i < E.words) E.p[i] = 0u;
+    }
+}
+
+int zero_batched_launch(const std::vector<std::pair<void *, uint32_t>> &buffers, GroupLaunch &L) {
+    L = GroupLaunch{};
+    L.kind = kGrpZero;
+    L.block = kThreads;
+    L.first.push_back(0);
+    for (const auto &b : buffers) {
+        if (!b.second) continue;
+        if (!b.first) { set_error("batched zero: null buffer"); return kErrArg; }
+        ZeroEntry E{static_cast<uint32_t *>(b.first), b.second, 0};
+        const size_t o = L.params.size();
+        L.params.resize(o + sizeof(E));
+        std::memcpy(L.params.data() + o, &E, sizeof(E));
+        L.first.push_back(L.first.back() + div_up(b.second, kThreads));
+        L.nent++;
+    }
+    L.total = L.first.back();
+    return kOk;
+}
+
+int bitplane_group_run(const GroupLaunch &L, hipStream_t s) {
+    if (L.kind != kGrpNbEncode && L.kind != kGrpZero) { set_error("grouped launch: not a bitplane kind"); return kErrArg; }
+    if (!L.d_params || !L.d_first) { set_error("grouped launch: tables not uploaded"); return kErrState; }
+    const uint32_t grid = L.grid();
+    if (!grid || !L.nent) return kOk;
+    if (L.kind == kGrpZero)
+        hipLaunchKernelGGL(k_zero_grouped, dim3(grid), dim3(L.block), 0, s, static_cast<const ZeroEntry *>(L.d_params),
+                           L.d_first, L.nent, L.total);
+    else if (L.elem == 8)
+        hipLaunchKernelGGL(k_nb_encode_grouped<double>, dim3(grid), dim3(L.block), 0, s,
+                           static_cast<const NbEntry<double> *>(L.d_params), L.d_first, L.nent, L.total, L.nb);
+    else
+        hipLaunchKernelGGL(k_nb_encode_grouped<float>, dim3(grid), dim3(L.block), 0, s,
+                           static_cast<const NbEntry<float> *>(L.d_params), L.d_first, L.nent, L.total, L.nb);
+    PRODM_CHECK_LAUNCH();
+    return kOk;
 }
 
 template <class T>

@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "internal.hpp"
+#include "plan.hpp"
 #include "prodm_mdr.h"
 
 namespace prodm {
@@ -255,35 +256,6 @@ static int collect_squared_impl(const T *x, uint64_t n, int nb, T maxerr, double
 
 }  // namespace prodm
 
-// ---- pipeline plan ------------------------------------------------------------------
-struct prodm_mdr_plan {
-    LevelGeom g;
-    int dtype = PRODM_F32, encoder = PRODM_ENC_NEGABINARY, nb = 32;
-    size_t elem = 4;
-    void *ws = nullptr;
-    size_t ws_bytes = 0;
-    void *levels = nullptr;          // interleaved level buffers
-    uint64_t level_off[33] = {0};    // element offsets
-    void *keys = nullptr;            // L+1 max keys
-    void *planes = nullptr;          // plane storage
-    uint64_t plane_off[33] = {0};    // word offsets per level
-    uint64_t plane_stride[33] = {0}; // words per plane
-    uint8_t *rb = nullptr;           // GroupedBP recording planes
-    uint64_t rb_off[33] = {0};
-    uint32_t *words = nullptr;       // GroupedBP words per plane, (L+1) * nb
-    void *signs = nullptr;           // GroupedBP decode signs
-    void *gbws = nullptr;
-    size_t gbws_bytes = 0;
-    uint64_t device_bytes = 0;
-    prodm::StageTimer timer;
-    int hier = 0;  // PRODM_DECOMP_HIERARCHICAL
-    // captured graphs: refactor of one input pointer, retrieve of one
-    // (planes per level, output pointer) pair
-    hipGraphExec_t g_ref = nullptr, g_ret = nullptr;
-    const void *g_ref_in = nullptr;
-    void *g_ret_out = nullptr;
-    uint8_t g_ret_k[33] = {0};
-};
 
 namespace prodm {
 
@@ -768,20 +740,6 @@ int prodm_mdr_set_level_max(prodm_mdr_plan *p, int level, double max_abs, prodm_
 }
 
 // Capture `body` on stream s into a graph executable (kernel nodes only).
-extern "C++" template <class F> static int capture(hipStream_t s, hipGraphExec_t *exec, F body) {
-    if (!s) { set_error("graph capture needs a non-default stream"); return kErrArg; }
-    if (*exec) { (void)hipGraphExecDestroy(*exec); *exec = nullptr; }
-    PRODM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    const int rc = body();
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(s, &g);
-    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
-    if (e != hipSuccess) { set_error(std::string("graph capture: ") + hipGetErrorString(e)); return kErrHip; }
-    const hipError_t ei = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (ei != hipSuccess) { *exec = nullptr; set_error(std::string("graph instantiate: ") + hipGetErrorString(ei)); return kErrHip; }
-    return kOk;
-}
 
 int prodm_mdr_refactor_graph(prodm_mdr_plan *p, const void *d_in, prodm_stream_t stream) {
     if (!p || !d_in) { set_error("refactor_graph: null argument"); return kErrArg; }
@@ -789,7 +747,7 @@ int prodm_mdr_refactor_graph(prodm_mdr_plan *p, const void *d_in, prodm_stream_t
     if (!p->g_ref || p->g_ref_in != d_in) {
         const bool on = p->timer.on;
         p->timer.on = false;  // no event nodes in the graph
-        const int rc = capture(s, &p->g_ref, [&] { return prodm_mdr_refactor(p, d_in, stream); });
+        const int rc = prodm::graph_capture(s, &p->g_ref, [&] { return prodm_mdr_refactor(p, d_in, stream); });
         p->timer.on = on;
         if (rc) return rc;
         p->g_ref_in = d_in;
@@ -804,7 +762,7 @@ int prodm_mdr_retrieve_graph(prodm_mdr_plan *p, const uint8_t *k, void *d_out, p
     if (!p->g_ret || p->g_ret_out != d_out || std::memcmp(p->g_ret_k, k, p->g.L + 1) != 0) {
         const bool on = p->timer.on;
         p->timer.on = false;
-        const int rc = capture(s, &p->g_ret, [&] { return prodm_mdr_retrieve(p, k, d_out, stream); });
+        const int rc = prodm::graph_capture(s, &p->g_ret, [&] { return prodm_mdr_retrieve(p, k, d_out, stream); });
         p->timer.on = on;
         if (rc) return rc;
         p->g_ret_out = d_out;

@@ -144,4 +144,38 @@ template <> __host__ __device__ inline double key_to_value<double>(uint64_t k) {
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// Global address space qualifier: accesses through a pointer read from
+// memory (not a kernel argument) are otherwise generic (flat_*); casting to
+// a PRODM_GLOBAL pointer makes them global_* (scalar base + 32-bit offset).
+#define PRODM_GLOBAL __attribute__((address_space(1)))
+// Constant address space: global memory that no wave writes while the
+// kernel runs (wave-uniform loads from it are scalar loads).
+#define PRODM_CONST __attribute__((address_space(4)))
+
+// A parameter struct at a wave-uniform address, copied into scalar
+// registers word by word (every lane reads the same words).
+template <class S> __device__ __forceinline__ S uniform_load(const S *p) {
+    static_assert(sizeof(S) % 4 == 0, "parameter structs are whole words");
+    constexpr int kW = sizeof(S) / 4;
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(p);
+    uint32_t w[kW];
+#pragma unroll
+    for (int i = 0; i < kW; i++) w[i] = (uint32_t)__builtin_amdgcn_readfirstlane((int)src[i]);
+    S out;
+    __builtin_memcpy(&out, w, sizeof(S));
+    return out;
+}
+
+// Entry of a grouped launch that owns virtual workgroup vb: entry e owns
+// [first[e], first[e + 1]), first[0] = 0, first[n] = total > vb.
+__device__ __forceinline__ uint32_t group_entry(const uint32_t *__restrict__ first, uint32_t n, uint32_t vb) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (first[mid] <= vb) lo = mid;
+        else hi = mid;
+    }
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)lo);
+}
+
 }  // namespace prodm

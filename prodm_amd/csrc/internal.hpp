@@ -119,4 +119,68 @@ int mgard_adjoint_coarse(const double *gfine, const LevelGeom &g, int l, double 
 // t = S h of level l (rows of `pitch` doubles) in the workspace
 const double *mgard_adjoint_solution(const LevelGeom &g, int l, void *ws, uint32_t *pitch);
 
+// ---- grouped launches (batched refactor) -----------------------------------
+// A grouped launch runs one kernel over the work of many fields: entry e
+// (one field's parameters for that kernel) owns the virtual workgroups
+// [first[e], first[e + 1]), and grid() workgroups walk all `total` of them
+// with stride grid() (a persistent loop when the grid is capped).
+enum GroupKind : int { kGrpLoad = 0, kGrpContig = 1, kGrpStrided = 2, kGrpLevel0 = 3, kGrpNbEncode = 4, kGrpZero = 5 };
+// Shape knobs of one grouped launch; none changes any result.
+struct GroupShape {
+    int a = 0;         // load: coarse planes per tile march (0: per-entry default);
+                       // contig solve: lines (waves) per workgroup (0: 4)
+    uint32_t cap = 0;  // workgroups of the launch (0: one per virtual workgroup)
+};
+struct GroupLaunch {
+    int kind = 0;
+    int elem = 4;       // bytes of the field type
+    int variant = 0;    // kernel instance (register segment length of a solve)
+    int nb = 0;         // encode: bitplanes
+    uint32_t block = 256, nent = 0, total = 0;
+    GroupShape shape;
+    std::vector<uint8_t> params;   // nent parameter structs of the kind
+    std::vector<uint32_t> first;   // nent + 1 prefix offsets
+    const void *d_params = nullptr;
+    const uint32_t *d_first = nullptr;
+    uint32_t grid() const {
+        uint64_t g = total;
+        if (shape.cap && shape.cap < g) g = ((uint64_t)shape.cap + 7) / 8 * 8;  // whole rounds over the 8 XCDs
+        const uint64_t lim = 0xffffffffull / block;
+        return (uint32_t)(g < lim ? g : lim / 8 * 8);
+    }
+};
+// One field of a batched decompose: dense input, geometry, interleaved level
+// buffers, L+1 max keys and its own MGARD workspace (factor tables uploaded).
+struct BatchFieldRef {
+    const void *in = nullptr;
+    const LevelGeom *g = nullptr;
+    void *level[33] = {nullptr};
+    void *keys = nullptr;
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    bool hier = false;
+};
+// The grouped launches of a batched decompose, in stream order: per level
+// step j (level L_f - j of every field with L_f - j >= 1) the load sweep and
+// the line solves along dims 2, 1, 0, then the coarsest grids. shapes[i]
+// shapes launch i (missing entries: defaults). Every field needs L >= 1.
+template <class T>
+int mgard_batched_launches(const std::vector<BatchFieldRef> &fields, const std::vector<GroupShape> &shapes,
+                           std::vector<GroupLaunch> &out);
+int mgard_group_run(const GroupLaunch &L, hipStream_t s);
+// One level of a batched NegaBinary encode (planes of level buffer x).
+struct EncodeRef {
+    const void *x = nullptr;
+    uint32_t n = 0;
+    void *key = nullptr;
+    void *planes = nullptr;
+    uint64_t stride = 0;
+};
+template <class T>
+int nb_batched_launch(const std::vector<EncodeRef> &levels, int nb, const GroupShape &shape, GroupLaunch &out);
+// Zeroes many small buffers (u32 word counts) in one launch: the level max
+// keys of a batch before its decompose.
+int zero_batched_launch(const std::vector<std::pair<void *, uint32_t>> &buffers, GroupLaunch &out);
+int bitplane_group_run(const GroupLaunch &L, hipStream_t s);
+
 }  // namespace prodm

@@ -22,6 +22,8 @@
 #include "internal.hpp"
 
 #include <algorithm>
+#include <cstring>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -142,19 +144,30 @@ __device__ inline MarchTile march_tile(uint32_t bt, uint32_t n0, uint32_t n1, ui
     return t;
 }
 
+// workgroups (coarse tiles) of one load sweep
+template <class T> __host__ __device__ inline uint32_t load_tiles(const LoadParams<T> &P) {
+    const uint32_t t2n = (P.n[2] == 1) ? 1 : (P.c[2] + kM2 - 1) / kM2;
+    const uint32_t t1n = (P.n[1] == 1) ? 1 : (P.c[1] + kM1 - 1) / kM1;
+    const uint32_t t0n = (P.n[0] == 1) ? 1 : (P.c[0] + P.march - 1) / P.march;
+    return t0n * t1n * t2n;
+}
+
+// the sweep of coarse tile bt (one workgroup)
 template <class T, int MODE, bool ILV>
-__global__ __launch_bounds__(kLoadThreads) void k_load(LoadParams<T> P) {
+__device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt) {
     using K = KeyOf<T>;
+    // buffer accesses through global-address-space pointers: the parameters
+    // may come from memory (grouped launches), where the compiler cannot
+    // infer the address space and would emit flat_* operations
+    using GT = PRODM_GLOBAL T;
+    using GChar = PRODM_GLOBAL char;
     constexpr bool kDec = MODE == kDecompose;
     constexpr int kPE = kDec ? kPL : 1;  // input planes staged only by decompose
     __shared__ T Ea[kPE], Eb[kPE], O[kPE], Wb[kPL];
     __shared__ double G2[kG2];
     const int tid = threadIdx.x;
     const uint32_t n0 = P.n[0], n1 = P.n[1], n2 = P.n[2], c0 = P.c[0], c1 = P.c[1], c2 = P.c[2];
-    const uint32_t t2n = (n2 == 1) ? 1 : (c2 + kM2 - 1) / kM2;
-    const uint32_t t1n = (n1 == 1) ? 1 : (c1 + kM1 - 1) / kM1;
-    const uint32_t t0n = (n0 == 1) ? 1 : (c0 + P.march - 1) / P.march;
-    const MarchTile M = march_tile(xcd_remap(blockIdx.x, t0n * t1n * t2n), n0, n1, n2, c0, c1, c2, P.march);
+    const MarchTile M = march_tile(bt, n0, n1, n2, c0, c1, c2, P.march);
     const bool act0 = n0 > 1, act1 = n1 > 1, act2 = n2 > 1;
 
     // ---- per-slot constants: slot m of this thread is plane element
@@ -208,18 +221,17 @@ __global__ __launch_bounds__(kLoadThreads) void k_load(LoadParams<T> P) {
     auto fetch = [&](T (&reg)[kPER], int i0) __attribute__((always_inline)) {
         const bool ok = i0 >= 0 && i0 < (int)n0;
         const bool odd0 = act0 && (i0 & 1);
-        const char *pb = MODE == kRecompose
-                             ? reinterpret_cast<const char *>((ILV ? P.coef_ilv : P.coef_full) + coef_base(i0))
-                             : reinterpret_cast<const char *>(P.in + (uint64_t)i0 * P.is[0]);
+        const GChar *pb = MODE == kRecompose ? (const GChar *)((const GT *)(ILV ? P.coef_ilv : P.coef_full) + coef_base(i0))
+                                             : (const GChar *)((const GT *)P.in + (uint64_t)i0 * P.is[0]);
 #pragma unroll
         for (int m = 0; m < kPER; m++) {
             const uint32_t b = bits >> (4 * m);
             T v = (T)0;
             if (ok && (b & 1)) {
                 if (MODE == kRecompose) {
-                    if (odd0 || (b & 6)) v = *reinterpret_cast<const T *>(pb + (odd0 ? offB[m] : offA[m]));
+                    if (odd0 || (b & 6)) v = *(const GT *)(pb + (odd0 ? offB[m] : offA[m]));
                 } else {
-                    v = *reinterpret_cast<const T *>(pb + in_off[m]);
+                    v = *(const GT *)(pb + in_off[m]);
                 }
             }
             reg[m] = v;
@@ -256,8 +268,8 @@ __global__ __launch_bounds__(kLoadThreads) void k_load(LoadParams<T> P) {
         const bool o0 = act0 && (i0 & 1);
         const bool own0 = !act0 || (i0 >= 2 * M.js0 - 1 && i0 < 2 * (M.js0 + M.tj0) - 1);
         const T *PA = o0 ? A : C, *PB = o0 ? Bp : C;
-        char *cp = reinterpret_cast<char *>((ILV ? P.coef_ilv : P.coef_full) + coef_base(i0));
-        char *up = reinterpret_cast<char *>(P.uc + (uint64_t)(i0 >> 1) * c1 * c2);
+        GChar *cp = (GChar *)((GT *)(ILV ? P.coef_ilv : P.coef_full) + coef_base(i0));
+        GChar *up = (GChar *)((GT *)P.uc + (uint64_t)(i0 >> 1) * c1 * c2);
 #pragma unroll
         for (int m = 0; m < kPER; m++) {
             const int idx = tid + kLoadThreads * m;
@@ -280,8 +292,8 @@ __global__ __launch_bounds__(kLoadThreads) void k_load(LoadParams<T> P) {
             const bool st = (b & 1) && (b & 8) && own0;
             const uint32_t oa = offA[m], ob = offB[m];
             const uint32_t oc = o0 ? ob : oa;
-            char *dst = coarse ? up + oa : cp + oc;
-            if (st) *reinterpret_cast<T *>(dst) = coarse ? cv : w;
+            GChar *dst = coarse ? up + oa : cp + oc;
+            if (st) *(GT *)dst = coarse ? cv : w;
             const T aw = (st && !coarse && w == w) ? fabs(w) : (T)0;
             lmax = aw > lmax ? aw : lmax;
         }
@@ -332,7 +344,7 @@ __global__ __launch_bounds__(kLoadThreads) void k_load(LoadParams<T> P) {
     };
     auto in_tile0 = [&](int j0) __attribute__((always_inline)) -> bool { return j0 >= M.js0 && j0 < M.js0 + M.tj0; };
     auto store_f = [&](int j0, double v) __attribute__((always_inline)) {
-        P.f[((uint64_t)j0 * c1 + j1) * P.fp + j2] = v;
+        ((PRODM_GLOBAL double *)P.f)[((uint64_t)j0 * c1 + j1) * P.fp + j2] = v;
     };
     auto column = [&](int i0) __attribute__((always_inline)) {
         if (!has_col) return;
@@ -425,6 +437,11 @@ __global__ __launch_bounds__(kLoadThreads) void k_load(LoadParams<T> P) {
     }
 }
 
+template <class T, int MODE, bool ILV>
+__global__ __launch_bounds__(kLoadThreads) void k_load(LoadParams<T> P) {
+    load_tile<T, MODE, ILV>(P, xcd_remap(blockIdx.x, load_tiles(P)));
+}
+
 // ---- line solves -----------------------------------------------------------------
 // Thomas factors (scaled coarse mass matrix tridiag(1, 4, 1), ends 2) for a
 // line of m nodes, stored as inv[0..m) then cp[0..m).
@@ -451,19 +468,22 @@ template <class T> struct SolveParams {
 
 // fidx: position in f (padded rows), uidx: position in the dense u_c
 template <class T> __device__ inline void epilogue(const SolveParams<T> &S, uint64_t fidx, uint64_t uidx, double z) {
-    if (S.epi == kEpiAdd) S.uc_out[uidx] = (T)((double)S.uc_in[uidx] + z);
-    else if (S.epi == kEpiSub) S.uc_out[uidx] = (T)((double)S.uc_in[uidx] - z);
-    else S.f[fidx] = z;
+    using GT = PRODM_GLOBAL T;
+    if (S.epi == kEpiAdd) ((GT *)S.uc_out)[uidx] = (T)((double)((const GT *)S.uc_in)[uidx] + z);
+    else if (S.epi == kEpiSub) ((GT *)S.uc_out)[uidx] = (T)((double)((const GT *)S.uc_in)[uidx] - z);
+    else ((PRODM_GLOBAL double *)S.f)[fidx] = z;
 }
 
-// contiguous lines (dim 2): one wave per line, lane holds a segment of SEG
+// contiguous lines (dim 2): one wave per line, lane holds a segment of SEG;
+// wave w of the workgroup solves line line0 + w
 template <class T, int SEG>
-__global__ __launch_bounds__(256) void k_solve_contig(SolveParams<T> S, uint64_t nlines) {
+__device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint64_t nlines, uint64_t line0) {
     const uint32_t m = S.c[2];
     const uint32_t lane = lane_id(), wv = threadIdx.x / 64;
-    const uint64_t line = (uint64_t)blockIdx.x * 4 + wv;
+    const uint64_t line = line0 + wv;
     if (line >= nlines) return;
-    const double *x = S.f + line * S.fp;
+    const PRODM_GLOBAL double *x = (const PRODM_GLOBAL double *)S.f + line * S.fp;
+    const PRODM_GLOBAL double *inv = (const PRODM_GLOBAL double *)S.inv, *cpv = (const PRODM_GLOBAL double *)S.cp;
     const uint32_t seg = (m + 63) / 64;
     const uint32_t b = lane * seg;
     const uint32_t e = min(m, b + seg);
@@ -475,7 +495,7 @@ __global__ __launch_bounds__(256) void k_solve_contig(SolveParams<T> S, uint64_t
 #pragma unroll
     for (int i = 0; i < SEG; i++) {
         if (b + i < e) {
-            const double iv = S.inv[b + i];
+            const double iv = inv[b + i];
             A = (v[i] - A) * iv;
             Bm = -Bm * iv;
         }
@@ -489,7 +509,7 @@ __global__ __launch_bounds__(256) void k_solve_contig(SolveParams<T> S, uint64_t
 #pragma unroll
     for (int i = 0; i < SEG; i++) {
         if (b + i < e) {
-            y = (v[i] - y) * S.inv[b + i];
+            y = (v[i] - y) * inv[b + i];
             v[i] = y;
         }
     }
@@ -498,7 +518,7 @@ __global__ __launch_bounds__(256) void k_solve_contig(SolveParams<T> S, uint64_t
 #pragma unroll
     for (int i = SEG - 1; i >= 0; i--) {
         if (b + i < e) {
-            const double c = (b + i == m - 1) ? 0.0 : S.cp[b + i];
+            const double c = (b + i == m - 1) ? 0.0 : cpv[b + i];
             C = v[i] - c * C;
             Dm = -c * Dm;
         }
@@ -512,7 +532,7 @@ __global__ __launch_bounds__(256) void k_solve_contig(SolveParams<T> S, uint64_t
 #pragma unroll
     for (int i = SEG - 1; i >= 0; i--) {
         if (b + i < e) {
-            const double c = (b + i == m - 1) ? 0.0 : S.cp[b + i];
+            const double c = (b + i == m - 1) ? 0.0 : cpv[b + i];
             z = v[i] - c * z;
             v[i] = z;
         }
@@ -521,18 +541,25 @@ __global__ __launch_bounds__(256) void k_solve_contig(SolveParams<T> S, uint64_t
     for (int i = 0; i < SEG; i++)
         if (b + i < e) epilogue(S, line * S.fp + b + i, line * m + b + i, v[i]);
 }
+template <class T, int SEG>
+__global__ __launch_bounds__(256) void k_solve_contig(SolveParams<T> S, uint64_t nlines) {
+    solve_contig_lines<T, SEG>(S, nlines, (uint64_t)blockIdx.x * 4);
+}
 
 // strided lines along dim d (1 or 0) of the coarse grid, lane per line.
 // Lines are indexed over padded rows, q = r * fp + j2 (r = j0 for d = 1,
 // j1 for d = 0); lanes with j2 >= c2 idle, so each wave's 64 accesses are
-// one aligned 512-byte run of f. W waves per workgroup split the line into
-// segments of SEG.
+// one aligned 512-byte run of f. nw waves split the line into segments of
+// at most SEG (nw is a function of m alone, so the scan order is fixed);
+// waves of the workgroup past nw find empty segments and only join the
+// barriers. bid: the workgroup's 64-line group.
 template <class T, int SEG>
-__global__ __launch_bounds__(512) void k_solve_strided(SolveParams<T> S, uint32_t m, uint64_t nlines, int d) {
+__device__ __forceinline__ void solve_strided_lines(const SolveParams<T> &S, uint32_t m, uint64_t nlines, int d,
+                                                    uint32_t nw, uint32_t bid) {
     __shared__ double sA[8][64], sB[8][64];
     const uint32_t lane = lane_id();
-    const uint32_t wv = threadIdx.x / 64, nw = blockDim.x / 64;
-    const uint64_t q = (uint64_t)blockIdx.x * 64 + lane;
+    const uint32_t wv = threadIdx.x / 64;
+    const uint64_t q = (uint64_t)bid * 64 + lane;
     const uint32_t fp = S.fp, c1 = S.c[1], c2 = S.c[2];
     const uint32_t j2 = (uint32_t)(q % fp);
     const uint64_t r = q / fp;
@@ -547,18 +574,22 @@ __global__ __launch_bounds__(512) void k_solve_strided(SolveParams<T> S, uint32_
         ustride = (uint64_t)c1 * c2;
         if (act) { base = r * fp + j2; ubase = r * c2 + j2; }
     }
+    // the factors are read-only while a solve runs: constant-space pointers
+    // let their wave-uniform loads go through the scalar cache
+    const PRODM_GLOBAL double *fg = (const PRODM_GLOBAL double *)S.f;
+    const PRODM_CONST double *inv = (const PRODM_CONST double *)S.inv, *cpv = (const PRODM_CONST double *)S.cp;
     const uint32_t seg = (m + nw - 1) / nw;
     // wave-uniform segment bounds: the factor loads become scalar loads
     const uint32_t b = (uint32_t)__builtin_amdgcn_readfirstlane((int)(wv * seg));
     const uint32_t e = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(m, b + seg));
     double v[SEG];
 #pragma unroll
-    for (int i = 0; i < SEG; i++) v[i] = (act && b + i < e) ? S.f[base + (uint64_t)(b + i) * stride] : 0.0;
+    for (int i = 0; i < SEG; i++) v[i] = (act && b + i < e) ? fg[base + (uint64_t)(b + i) * stride] : 0.0;
     double A = 0.0, Bm = 1.0;
 #pragma unroll
     for (int i = 0; i < SEG; i++) {
         if (b + i < e) {
-            const double iv = S.inv[b + i];
+            const double iv = inv[b + i];
             A = (v[i] - A) * iv;
             Bm = -Bm * iv;
         }
@@ -572,7 +603,7 @@ __global__ __launch_bounds__(512) void k_solve_strided(SolveParams<T> S, uint32_
 #pragma unroll
     for (int i = 0; i < SEG; i++) {
         if (b + i < e) {
-            y = (v[i] - y) * S.inv[b + i];
+            y = (v[i] - y) * inv[b + i];
             v[i] = y;
         }
     }
@@ -580,7 +611,7 @@ __global__ __launch_bounds__(512) void k_solve_strided(SolveParams<T> S, uint32_
 #pragma unroll
     for (int i = SEG - 1; i >= 0; i--) {
         if (b + i < e) {
-            const double c = (b + i == m - 1) ? 0.0 : S.cp[b + i];
+            const double c = (b + i == m - 1) ? 0.0 : cpv[b + i];
             C = v[i] - c * C;
             Dm = -c * Dm;
         }
@@ -593,7 +624,7 @@ __global__ __launch_bounds__(512) void k_solve_strided(SolveParams<T> S, uint32_
 #pragma unroll
     for (int i = SEG - 1; i >= 0; i--) {
         if (b + i < e) {
-            const double c = (b + i == m - 1) ? 0.0 : S.cp[b + i];
+            const double c = (b + i == m - 1) ? 0.0 : cpv[b + i];
             z = v[i] - c * z;
             v[i] = z;
         }
@@ -603,6 +634,10 @@ __global__ __launch_bounds__(512) void k_solve_strided(SolveParams<T> S, uint32_
         for (int i = 0; i < SEG; i++)
             if (b + i < e) epilogue(S, base + (uint64_t)(b + i) * stride, ubase + (uint64_t)(b + i) * ustride, v[i]);
     }
+}
+template <class T, int SEG>
+__global__ __launch_bounds__(512) void k_solve_strided(SolveParams<T> S, uint32_t m, uint64_t nlines, int d) {
+    solve_strided_lines<T, SEG>(S, m, nlines, d, blockDim.x / 64, blockIdx.x);
 }
 
 // ---- recompose interpolation: u = I u_c + W ------------------------------------
@@ -715,6 +750,91 @@ static int copy_box(const T *src, uint64_t ss0, uint64_t ss1, T *dst, uint64_t d
     hipLaunchKernelGGL(k_copy_box<T>, grid, dim3(256), 0, s, src, ss0, ss1, dst, ds0, ds1, n[2]);
     PRODM_CHECK_LAUNCH();
     return kOk;
+}
+
+// ---- grouped kernels (batched refactor) ---------------------------------------
+// Each walks the virtual workgroups of a grouped launch (see GroupLaunch):
+// the entry's parameters are copied to scalar registers and the same tile /
+// line body as the single-field kernel runs, so every field's results equal
+// its single-field decompose bit for bit.
+template <class T> struct SolveEntry {
+    SolveParams<T> S;
+    uint64_t nlines;
+    uint32_t m, nw;   // line length, waves per line (strided)
+    int32_t d, pad;
+};
+template <class T> struct Level0Entry {
+    const T *src;     // coarsest grid (strides s0, s1)
+    uint64_t s0, s1;
+    T *dst;           // dense level-0 buffer
+    KeyOf<T> *key;
+    uint32_t c0, c1, c2, pad;
+};
+
+template <class T>
+__global__ __launch_bounds__(kLoadThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_load_grouped(const LoadParams<T> *__restrict__ ps,
+                                                               const uint32_t *__restrict__ first, uint32_t nent,
+                                                               uint32_t total) {
+    for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const uint32_t e = group_entry(first, nent, vb);
+        const LoadParams<T> &P = ps[e];
+        const uint32_t f0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)first[e]);
+        load_tile<T, kDecompose, true>(P, xcd_remap(vb - f0, load_tiles(P)));
+        __syncthreads();  // LDS planes are reused by the next tile
+    }
+}
+
+template <class T, int SEG>
+__global__ __launch_bounds__(512) void k_contig_grouped(const SolveEntry<T> *__restrict__ ps,
+                                                        const uint32_t *__restrict__ first, uint32_t nent,
+                                                        uint32_t total) {
+    const uint32_t wpb = blockDim.x / 64;
+    for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const uint32_t e = group_entry(first, nent, vb);
+        const SolveEntry<T> E = uniform_load(ps + e);
+        const uint32_t f0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)first[e]);
+        solve_contig_lines<T, SEG>(E.S, E.nlines, (uint64_t)(vb - f0) * wpb);
+    }
+}
+
+template <class T, int SEG>
+__global__ __launch_bounds__(512) void k_strided_grouped(const SolveEntry<T> *__restrict__ ps,
+                                                         const uint32_t *__restrict__ first, uint32_t nent,
+                                                         uint32_t total) {
+    for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const uint32_t e = group_entry(first, nent, vb);
+        const SolveEntry<T> E = uniform_load(ps + e);
+        const uint32_t f0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)first[e]);
+        solve_strided_lines<T, SEG>(E.S, E.m, E.nlines, E.d, E.nw, vb - f0);
+        __syncthreads();  // the cross-wave scan buffers are reused
+    }
+}
+
+// coarsest grid of every field: dense copy into the level-0 buffer and its
+// max |value| (NaN excluded), as copy_box + k_maxabs do for one field
+template <class T>
+__global__ __launch_bounds__(256) void k_level0_grouped(const Level0Entry<T> *__restrict__ ps,
+                                                        const uint32_t *__restrict__ first, uint32_t nent,
+                                                        uint32_t total) {
+    using K = KeyOf<T>;
+    for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const uint32_t e = group_entry(first, nent, vb);
+        const Level0Entry<T> E = uniform_load(ps + e);
+        const uint32_t f0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)first[e]);
+        const uint32_t idx = (vb - f0) * 256 + threadIdx.x;
+        K m = 0;
+        if (idx < E.c0 * E.c1 * E.c2) {
+            const uint32_t i2 = idx % E.c2, i1 = (idx / E.c2) % E.c1, i0 = idx / (E.c2 * E.c1);
+            const T v = ((const PRODM_GLOBAL T *)E.src)[(uint64_t)i0 * E.s0 + (uint64_t)i1 * E.s1 + i2];
+            ((PRODM_GLOBAL T *)E.dst)[idx] = v;
+            if (v == v) m = abs_key(v);
+        }
+        for (int o = 32; o >= 1; o >>= 1) {
+            const K q = __shfl_xor(m, o, 64);
+            m = q > m ? q : m;
+        }
+        if (lane_id() == 0 && m) atomicMax(E.key, m);
+    }
 }
 
 // ---- host orchestration -----------------------------------------------------------
@@ -1018,5 +1138,229 @@ template int mgard_recompose<float>(const CoefSink<float> &, const LevelGeom &, 
                                     size_t, hipStream_t, StageTimer *, bool);
 template int mgard_recompose<double>(const CoefSink<double> &, const LevelGeom &, int, double *, const uint64_t *,
                                      void *, size_t, hipStream_t, StageTimer *, bool);
+
+// ---- batched decompose: grouped launches over many fields ---------------------
+namespace {
+
+template <class E> void append_entry(GroupLaunch &L, const E &e, uint32_t blocks) {
+    const size_t o = L.params.size();
+    L.params.resize(o + sizeof(E));
+    std::memcpy(L.params.data() + o, &e, sizeof(E));
+    if (L.first.empty()) L.first.push_back(0);
+    L.first.push_back(L.first.back() + blocks);
+    L.nent++;
+    L.total = L.first.back();
+}
+
+// entries in decreasing work, so that the largest fields are dispatched first
+template <class E> void fill_sorted(GroupLaunch &L, std::vector<std::pair<uint32_t, E>> &v) {
+    std::stable_sort(v.begin(), v.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+    for (auto &x : v)
+        if (x.first) append_entry(L, x.second, x.first);
+}
+
+int seg_class_contig(uint32_t m) {
+    const uint32_t seg = (m + 63) / 64;
+    return seg <= 4 ? 4 : seg <= 8 ? 8 : seg <= 16 ? 16 : 0;
+}
+int seg_class_strided(uint32_t seg) {
+    return seg <= 8 ? 8 : seg <= 16 ? 16 : seg <= 24 ? 24 : seg <= 40 ? 40 : seg <= 72 ? 72 : 0;
+}
+uint32_t strided_waves(uint32_t m) { return std::max<uint32_t>(1, std::min<uint32_t>(8, (m + 15) / 16)); }
+
+template <class T> int default_march(const LoadParams<T> &P) {
+    // launch_load's rule: shorter marches while the level has few tiles
+    const uint32_t t2n = (P.n[2] == 1) ? 1 : div_up(P.c[2], kM2);
+    const uint32_t t1n = (P.n[1] == 1) ? 1 : div_up(P.c[1], kM1);
+    int march = 16;
+    while (march > 2 && (uint64_t)t2n * t1n * div_up(P.c[0], march) < 2048) march /= 2;
+    return march;
+}
+
+}  // namespace
+
+template <class T>
+int mgard_batched_launches(const std::vector<BatchFieldRef> &fields, const std::vector<GroupShape> &shapes,
+                           std::vector<GroupLaunch> &out) {
+    out.clear();
+    struct State {
+        const T *cur;
+        uint64_t cs0, cs1;
+        int which;
+    };
+    std::vector<State> st(fields.size());
+    int Lmax = 0;
+    for (size_t i = 0; i < fields.size(); i++) {
+        const BatchFieldRef &f = fields[i];
+        if (!f.g || !f.g->ok || f.g->L < 1 || !f.in || !f.ws || !f.keys) {
+            set_error("batched decompose: field " + std::to_string(i) + " needs a valid geometry with L >= 1");
+            return kErrArg;
+        }
+        if (f.ws_bytes < ws_layout(*f.g, sizeof(T)).total) {
+            set_error("batched decompose: workspace too small for field " + std::to_string(i));
+            return kErrArg;
+        }
+        Lmax = std::max(Lmax, f.g->L);
+        st[i] = State{static_cast<const T *>(f.in), (uint64_t)f.g->dims[1] * f.g->dims[2], f.g->dims[2], 0};
+    }
+    auto make = [&](int kind, uint32_t block) {
+        GroupLaunch L;
+        L.kind = kind;
+        L.elem = sizeof(T);
+        L.block = block;
+        if (out.size() < shapes.size()) L.shape = shapes[out.size()];
+        return L;
+    };
+    for (int j = 0; j < Lmax; j++) {
+        GroupLaunch load = make(kGrpLoad, kLoadThreads);
+        std::vector<std::pair<uint32_t, LoadParams<T>>> le;
+        std::vector<std::pair<uint32_t, SolveEntry<T>>> se[3];
+        for (size_t i = 0; i < fields.size(); i++) {
+            const BatchFieldRef &f = fields[i];
+            const LevelGeom &g = *f.g;
+            const int l = g.L - j;
+            if (l < 1) continue;
+            const WsLayout w = ws_layout(g, sizeof(T));
+            char *base = static_cast<char *>(f.ws);
+            T *bufs[2] = {reinterpret_cast<T *>(base + w.xa), reinterpret_cast<T *>(base + w.xb)};
+            double *fv = reinterpret_cast<double *>(base + w.f);
+            const double *tab = reinterpret_cast<const double *>(base + w.tab);
+            CoefSink<T> sink;
+            for (int k = 0; k <= g.L; k++) sink.level[k] = static_cast<T *>(f.level[k]);
+            State &s = st[i];
+            LoadParams<T> P = load_params(g, l, sink);
+            P.in = s.cur;
+            P.is[0] = s.cs0; P.is[1] = s.cs1; P.is[2] = 1;
+            T *uc = bufs[s.which];
+            P.uc = uc;
+            P.f = fv;
+            P.fp = f_pitch(P.c[2]);
+            P.key = static_cast<KeyOf<T> *>(f.keys) + l;
+            P.hb = f.hier;
+            P.march = load.shape.a > 0 ? load.shape.a : default_march(P);
+            le.push_back({load_tiles(P), P});
+            if (!f.hier) {
+                const uint32_t *c = g.ld[l - 1], *n = g.ld[l];
+                int last = -1;
+                for (int d = 0; d < 3; d++)
+                    if (n[d] > 1) { last = d; break; }
+                const Factors F = make_factors(g);
+                for (int d = 2; d >= 0; d--) {
+                    if (n[d] == 1) continue;
+                    SolveEntry<T> E{};
+                    E.S.f = fv;
+                    E.S.fp = f_pitch(c[2]);
+                    E.S.c[0] = c[0]; E.S.c[1] = c[1]; E.S.c[2] = c[2];
+                    E.S.inv = tab + F.off[l][d];
+                    E.S.cp = E.S.inv + c[d];
+                    E.S.epi = (d == last) ? kEpiAdd : kEpiNone;
+                    E.S.uc_in = uc;
+                    E.S.uc_out = uc;
+                    E.m = c[d];
+                    E.d = d;
+                    E.nlines = d == 2 ? (uint64_t)c[0] * c[1] : (uint64_t)(d == 1 ? c[0] : c[1]) * E.S.fp;
+                    E.nw = d == 2 ? 1 : strided_waves(E.m);
+                    se[d].push_back({0, E});
+                }
+            }
+            s.cur = uc;
+            s.cs0 = (uint64_t)g.ld[l - 1][1] * g.ld[l - 1][2];
+            s.cs1 = g.ld[l - 1][2];
+            s.which ^= 1;
+        }
+        fill_sorted(load, le);
+        out.push_back(std::move(load));
+        // line solves along dims 2, 1, 0
+        for (int d = 2; d >= 0; d--) {
+            if (se[d].empty()) continue;
+            if (d == 2) {
+                GroupLaunch L = make(kGrpContig, 256);
+                const uint32_t wpb = L.shape.a > 0 ? std::min(8, L.shape.a) : 4;
+                L.block = 64 * wpb;
+                uint32_t mmax = 0;
+                for (auto &x : se[d]) {
+                    mmax = std::max(mmax, x.second.m);
+                    x.first = div_up(x.second.nlines, wpb);
+                }
+                L.variant = seg_class_contig(mmax);
+                if (!L.variant) { set_error("line too long for the contiguous solve"); return kErrUnsupported; }
+                fill_sorted(L, se[d]);
+                out.push_back(std::move(L));
+            } else {
+                GroupLaunch L = make(kGrpStrided, 64);
+                uint32_t segmax = 0, nwmax = 1;
+                for (auto &x : se[d]) {
+                    const uint32_t nw = x.second.nw;
+                    segmax = std::max(segmax, (x.second.m + nw - 1) / nw);
+                    nwmax = std::max(nwmax, nw);
+                    x.first = div_up(x.second.nlines, 64);
+                }
+                L.variant = seg_class_strided(segmax);
+                if (!L.variant) { set_error("line too long for the strided solve"); return kErrUnsupported; }
+                L.block = 64 * nwmax;
+                fill_sorted(L, se[d]);
+                out.push_back(std::move(L));
+            }
+        }
+    }
+    // coarsest grids
+    GroupLaunch L0 = make(kGrpLevel0, 256);
+    std::vector<std::pair<uint32_t, Level0Entry<T>>> e0;
+    for (size_t i = 0; i < fields.size(); i++) {
+        const LevelGeom &g = *fields[i].g;
+        Level0Entry<T> E{};
+        E.src = st[i].cur;
+        E.s0 = st[i].cs0;
+        E.s1 = st[i].cs1;
+        E.dst = static_cast<T *>(fields[i].level[0]);
+        E.key = static_cast<KeyOf<T> *>(fields[i].keys);
+        E.c0 = g.ld[0][0]; E.c1 = g.ld[0][1]; E.c2 = g.ld[0][2];
+        e0.push_back({div_up(points3(g.ld[0]), 256), E});
+    }
+    fill_sorted(L0, e0);
+    out.push_back(std::move(L0));
+    return kOk;
+}
+template int mgard_batched_launches<float>(const std::vector<BatchFieldRef> &, const std::vector<GroupShape> &,
+                                           std::vector<GroupLaunch> &);
+template int mgard_batched_launches<double>(const std::vector<BatchFieldRef> &, const std::vector<GroupShape> &,
+                                            std::vector<GroupLaunch> &);
+
+namespace {
+template <class T> int group_run_t(const GroupLaunch &L, hipStream_t s) {
+    const uint32_t grid = L.grid();
+    if (!grid || !L.nent) return kOk;
+    const uint32_t *first = L.d_first;
+    const dim3 G(grid), B(L.block);
+#define PRODM_GRP(kern, PT) hipLaunchKernelGGL(kern, G, B, 0, s, static_cast<const PT *>(L.d_params), first, L.nent, L.total)
+    switch (L.kind) {
+    case kGrpLoad: PRODM_GRP((k_load_grouped<T>), LoadParams<T>); break;
+    case kGrpContig:
+        if (L.variant == 4) PRODM_GRP((k_contig_grouped<T, 4>), SolveEntry<T>);
+        else if (L.variant == 8) PRODM_GRP((k_contig_grouped<T, 8>), SolveEntry<T>);
+        else if (L.variant == 16) PRODM_GRP((k_contig_grouped<T, 16>), SolveEntry<T>);
+        else { set_error("grouped contiguous solve: bad variant"); return kErrArg; }
+        break;
+    case kGrpStrided:
+        if (L.variant == 8) PRODM_GRP((k_strided_grouped<T, 8>), SolveEntry<T>);
+        else if (L.variant == 16) PRODM_GRP((k_strided_grouped<T, 16>), SolveEntry<T>);
+        else if (L.variant == 24) PRODM_GRP((k_strided_grouped<T, 24>), SolveEntry<T>);
+        else if (L.variant == 40) PRODM_GRP((k_strided_grouped<T, 40>), SolveEntry<T>);
+        else if (L.variant == 72) PRODM_GRP((k_strided_grouped<T, 72>), SolveEntry<T>);
+        else { set_error("grouped strided solve: bad variant"); return kErrArg; }
+        break;
+    case kGrpLevel0: PRODM_GRP((k_level0_grouped<T>), Level0Entry<T>); break;
+    default: set_error("grouped launch: not an MGARD kind"); return kErrArg;
+    }
+#undef PRODM_GRP
+    PRODM_CHECK_LAUNCH();
+    return kOk;
+}
+}  // namespace
+
+int mgard_group_run(const GroupLaunch &L, hipStream_t s) {
+    if (!L.d_params || !L.d_first) { set_error("grouped launch: tables not uploaded"); return kErrState; }
+    return L.elem == 8 ? group_run_t<double>(L, s) : group_run_t<float>(L, s);
+}
 
 }  // namespace prodm

@@ -325,7 +325,21 @@ class Plan:
         check(lib().prodm_mdr_stage_times(self.h, ms, cnt, n), "stage times")
         return {capi.STAGES[i]: (ms[i], cnt[i]) for i in range(n)}
 
+    @classmethod
+    def borrowed(cls, handle, dims, L, nb, dtype, encoder, owner):
+        """A Plan view of a handle owned elsewhere (a batch field); `owner`
+        is kept alive and the handle is never destroyed through the view."""
+        self = cls.__new__(cls)
+        self.dims, self.L, self.nb, self.dtype, self.encoder = list(dims), L, nb, dtype, encoder
+        self.h = C.c_void_p(handle)
+        self._owner = owner
+        self.level_dims, self.level_elements, _ = level_geometry(dims, L)
+        return self
+
     def close(self):
+        if getattr(self, "_owner", None) is not None:
+            self.h = None
+            return
         if getattr(self, "h", None):
             lib().prodm_mdr_plan_destroy(self.h)
             self.h = None

@@ -1,0 +1,165 @@
+"""GPU parity of the batched refactor (prodm_mdr_batch): every field's level
+coefficients, level maxima and bitplanes are bit-identical to a single-field
+plan refactor of the same input, for eager grouped launches, graph replays,
+autotuned shapes and cached shapes; the 96-field workload of 2-D / 3-D
+fields of uneven shapes is checked in full."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import bench
+from fields import bumps_field
+
+pytestmark = pytest.mark.gpu
+
+TD = {np.float32: torch.float32, np.float64: torch.float64}
+
+MIXED = [  # (dims, L): 2-D and 3-D fields of uneven shapes
+    ([65, 65, 65], 5), ([33, 129], 4), ([129, 33, 65], 4), ([257, 257], 6), ([33, 33, 33], 4),
+    ([65, 257], 5), ([129, 129, 33], 4), ([513, 65], 5), ([17, 17, 17], 3), ([65, 33, 129], 4),
+]
+
+
+def level_block(plan, level, nb, elem):
+    """the level's plane storage (nb planes of `stride` words) as a tensor,
+    plus the words per plane that hold data"""
+    from prodm_amd import mdr
+    p0, nbytes = plan.plane(level, 0)
+    stride = (plan.plane(level, 1)[0] - p0) if nb > 1 else nbytes
+    buf = torch.empty(nb * stride, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    mdr._d2d(buf, p0, nb * stride)
+    return buf.view(nb, stride)[:, :nbytes], nbytes
+
+
+def assert_same_results(bplan, pplan, L, nb, elem, encoder, what):
+    for l in range(L + 1):
+        assert bplan.level_info(l) == pplan.level_info(l), (what, l)
+        cb, cp = bplan.coefficients(l), pplan.coefficients(l)
+        assert torch.equal(cb.view(torch.uint8), cp.view(torch.uint8)), (what, "coefficients", l)
+        if encoder == 0:
+            a, na = level_block(bplan, l, nb, elem)
+            b, nbb = level_block(pplan, l, nb, elem)
+            assert na == nbb and torch.equal(a, b), (what, "planes", l)
+        else:
+            for p in range(nb):
+                assert bplan.plane_bytes(l, p) == pplan.plane_bytes(l, p), (what, l, p)
+            assert np.array_equal(bplan.level_rb(l), pplan.level_rb(l)), (what, "rb", l)
+
+
+def make_inputs(gpu, shapes, dt, seed0):
+    return [torch.from_numpy(bumps_field(d, seed=seed0 + i, dtype=dt)).to(gpu) for i, (d, L) in enumerate(shapes)]
+
+
+def plain_plans(xs, shapes, nb, dt, enc, decomposer=0):
+    from prodm_amd import mdr
+    plans = []
+    for x, (d, L) in zip(xs, shapes):
+        p = mdr.Plan(d, L, nb, TD[dt], enc, decomposer)
+        p.refactor(x)
+        plans.append(p)
+    torch.cuda.synchronize()
+    return plans
+
+
+@pytest.mark.parametrize("dt,enc,nb,decomp", [
+    (np.float32, 0, 32, 0),
+    (np.float32, 1, 32, 0),
+    (np.float64, 0, 64, 0),
+    (np.float64, 1, 60, 0),
+    (np.float32, 0, 32, 1),
+])
+def test_batched_equals_plain(gpu, dt, enc, nb, decomp):
+    from prodm_amd.batched import Batch
+    xs = make_inputs(gpu, MIXED, dt, 100)
+    plans = plain_plans(xs, MIXED, nb, dt, enc, decomp)
+    b = Batch([(x, d, L) for x, (d, L) in zip(xs, MIXED)], nb, TD[dt], enc, decomp)
+    b.refactor()
+    torch.cuda.synchronize()
+    desc = b.describe().strip().splitlines()
+    # grouped: far fewer launches than the fields' single-field sequences
+    assert len(desc) <= 4 * max(L for _, L in MIXED) + 3, desc
+    elem = 4 if dt == np.float32 else 8
+    for f, (d, L) in enumerate(MIXED):
+        assert_same_results(b.plan(f), plans[f], L, nb, elem, enc, f"field {f} {d}")
+
+
+def test_batched_graph_autotune_and_cache(gpu, tmp_path):
+    from prodm_amd.batched import Batch
+    shapes = MIXED[:7]
+    xs = make_inputs(gpu, shapes, np.float32, 200)
+    b = Batch([(x, d, L) for x, (d, L) in zip(xs, shapes)], 32)
+    s = torch.cuda.Stream()
+    cache = str(tmp_path / "shapes.txt")
+    with torch.cuda.stream(s):
+        b.autotune(cache, reps=2)
+        tuned = b.describe()
+        # graph replays, with the inputs changed in place between them
+        for rep in range(3):
+            for i, x in enumerate(xs):
+                x.copy_(torch.from_numpy(bumps_field(shapes[i][0], seed=300 + 10 * rep + i, dtype=np.float32)))
+            b.refactor_graph()
+            s.synchronize()
+            plans = plain_plans(xs, shapes, 32, np.float32, 0)
+            for f, (d, L) in enumerate(shapes):
+                assert_same_results(b.plan(f), plans[f], L, 32, 4, 0, f"replay {rep} field {f}")
+    assert os.path.exists(cache)
+    text = open(cache).read()
+    assert text.startswith("prodm-batch-shapes 1")
+    # a new batch over the same shapes reads the cache and launches the same shapes
+    b2 = Batch([(x, d, L) for x, (d, L) in zip(xs, shapes)], 32)
+    b2.autotune(cache)
+    assert b2.describe() == tuned
+    b2.refactor()
+    torch.cuda.synchronize()
+    for f, (d, L) in enumerate(shapes):
+        assert_same_results(b2.plan(f), b.plan(f), L, 32, 4, 0, f"cached field {f}")
+
+
+def test_batched_retrieve_through_field_plan(gpu):
+    from prodm_amd.batched import Batch
+    shapes = MIXED[:4]
+    xs = make_inputs(gpu, shapes, np.float32, 400)
+    plans = plain_plans(xs, shapes, 32, np.float32, 0)
+    b = Batch([(x, d, L) for x, (d, L) in zip(xs, shapes)], 32)
+    b.refactor()
+    for f, (d, L) in enumerate(shapes):
+        k = [min(32, 6 + 3 * l) for l in range(L + 1)]
+        ob = b.plan(f).retrieve(k)
+        op = plans[f].retrieve(k)
+        torch.cuda.synchronize()
+        assert torch.equal(ob, op), f
+
+
+def test_batched_rejects_bad_fields(gpu):
+    from prodm_amd import capi
+    from prodm_amd.batched import Batch
+    x = torch.zeros(33 * 33, device=gpu)
+    with pytest.raises(capi.ProdmError):
+        Batch([(x, [33, 33], 0)], 32)          # target level 0
+    with pytest.raises(capi.ProdmError):
+        Batch([(x, [33, 33], 9)], 32)          # too many levels for the shape
+    with pytest.raises(ValueError):
+        Batch([(x, [33, 34], 4)], 32)          # element count mismatch
+
+
+def test_batched_full_workload(gpu):
+    """96 fields (47 2-D, 49 3-D), 203 M points: bit-identical to plain runs"""
+    from prodm_amd import mdr
+    from prodm_amd.batched import Batch
+    shapes = bench.batch_shapes()
+    assert len(shapes) == 96
+    xs = [bench.make_batch_field_gpu(d, 1000 + i) for i, (d, L) in enumerate(shapes)]
+    b = Batch([(x, d, L) for x, (d, L) in zip(xs, shapes)], 32)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        b.refactor_graph()
+    s.synchronize()
+    for f, (d, L) in enumerate(shapes):
+        p = mdr.Plan(d, L, 32, torch.float32, 0)
+        p.refactor(xs[f])
+        torch.cuda.synchronize()
+        assert_same_results(b.plan(f), p, L, 32, 4, 0, f"field {f} {d}")
+        p.close()
