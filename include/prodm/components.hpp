@@ -183,6 +183,21 @@ class GpuNegaBinaryBPEncoder {
         }
         return streams;
     }
+    // encode with the per-plane level errors (NegaBinaryBPEncoder.hpp:65-125):
+    // level_errors[j] = squared error of keeping the first j planes. fp32
+    // only, as the reference's collect_level_errors (:231-239).
+    std::vector<uint8_t *> encode(T const *data, int32_t n, int32_t exp, uint8_t num_bitplanes,
+                                  std::vector<uint32_t> &stream_sizes, std::vector<double> &level_errors) const {
+        static_assert(std::is_same<T, float>::value, "the encoder's level errors are defined for float data");
+        std::vector<uint8_t *> streams = encode(data, n, exp, num_bitplanes, stream_sizes);
+        DeviceContext &c = *ctx_;
+        const size_t wsb = prodm_level_errors_workspace_bytes((uint64_t)n, num_bitplanes);
+        level_errors.assign(num_bitplanes + 1, 0.0);
+        check(prodm_negabinary_level_errors_f32(c.a.as<float>(n), (uint64_t)n, exp, num_bitplanes,
+                                                level_errors.data(), c.ws.get(wsb), wsb, c.stream),
+              "negabinary level errors");
+        return streams;
+    }
     T *decode(const std::vector<uint8_t const *> &streams, int32_t n, int exp, uint8_t num_bitplanes) {
         return progressive_decode(streams, n, exp, 0, num_bitplanes, 0);
     }
@@ -259,6 +274,21 @@ class GpuGroupedBPEncoder {
             streams[p] = buf;
             stream_sizes[p] = (uint32_t)(head + body);
         }
+        return streams;
+    }
+    // encode with the per-plane level errors (GroupedBPEncoder.hpp:76-145):
+    // level_errors[j] = squared error of keeping the first j planes. fp32
+    // only, as the reference's collect_level_errors (:324-335).
+    std::vector<uint8_t *> encode(T const *data, int32_t n, int32_t exp, uint8_t num_bitplanes,
+                                  std::vector<uint32_t> &stream_sizes, std::vector<double> &level_errors) const {
+        static_assert(std::is_same<T, float>::value, "the encoder's level errors are defined for float data");
+        std::vector<uint8_t *> streams = encode(data, n, exp, num_bitplanes, stream_sizes);
+        DeviceContext &c = *ctx_;
+        const size_t wsb = prodm_level_errors_workspace_bytes((uint64_t)n, num_bitplanes);
+        level_errors.assign(num_bitplanes + 1, 0.0);
+        check(prodm_grouped_level_errors_f32(c.a.as<float>(n), (uint64_t)n, exp, num_bitplanes, level_errors.data(),
+                                             c.ws.get(wsb), wsb, c.stream),
+              "grouped level errors");
         return streams;
     }
     T *decode(const std::vector<uint8_t const *> &streams, int32_t n, int exp, uint8_t num_bitplanes) {

@@ -161,6 +161,21 @@ int prodm_collect_squared_f32(const float *d_x, uint64_t n, int num_bitplanes, f
 int prodm_collect_squared_f64(const double *d_x, uint64_t n, int num_bitplanes, double max_level_error, double *h_out,
                               void *d_ws, size_t ws_bytes, prodm_stream_t stream);
 
+/* ---- encoder-fused level errors: the level_errors of
+ *      NegaBinaryBPEncoder::encode(..., level_errors)
+ *      (NegaBinaryBPEncoder.hpp:65-125, collect_level_errors :231-239) and
+ *      GroupedBPEncoder::encode(..., level_errors) (GroupedBPEncoder.hpp:
+ *      76-145, :324-335): entry j is the squared error of keeping the first
+ *      j planes of the codec. fp32 data, num_bitplanes <= 32 (the reference
+ *      collectors take float / uint32); exp as passed to the encoder. Each
+ *      entry is the exactly rounded sum of the reference's per-element terms.
+ *      Synchronizes the stream; h_out[0..nb].                               */
+size_t prodm_level_errors_workspace_bytes(uint64_t n, int num_bitplanes);
+int prodm_negabinary_level_errors_f32(const float *d_x, uint64_t n, int32_t exp, int num_bitplanes, double *h_out,
+                                      void *d_ws, size_t ws_bytes, prodm_stream_t stream);
+int prodm_grouped_level_errors_f32(const float *d_x, uint64_t n, int32_t exp, int num_bitplanes, double *h_out,
+                                   void *d_ws, size_t ws_bytes, prodm_stream_t stream);
+
 /* ---- device-resident refactor / retrieve pipeline: the GPU stages of
  *      ComposedRefactor::refactor (ComposedRefactor.hpp:78-141: decompose,
  *      interleave, max-abs, encode) and of ComposedReconstructor::reconstruct
@@ -177,6 +192,9 @@ uint64_t prodm_mdr_plan_device_bytes(const prodm_mdr_plan *plan);
 int prodm_mdr_refactor(prodm_mdr_plan *plan, const void *d_in, prodm_stream_t stream);
 /* level results (synchronizes): max |coef|, exponent, element count      */
 int prodm_mdr_level_info(prodm_mdr_plan *plan, int level, double *max_abs, int32_t *exp, uint64_t *elements);
+/* the plan encoder's level-error table of level l (fp32 plans; see
+ * prodm_negabinary_level_errors_f32); synchronizes                        */
+int prodm_mdr_level_errors(prodm_mdr_plan *plan, int level, double *h_out, prodm_stream_t stream);
 /* device pointer and byte size of plane p of level l (reference stream
  * bytes without the GroupedBP stream-0 header); synchronizes for GroupedBP */
 int prodm_mdr_plane(prodm_mdr_plan *plan, int level, int plane, const void **d_ptr, uint64_t *bytes);

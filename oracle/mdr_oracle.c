@@ -261,3 +261,101 @@ int orc_gb_state_level(orc_gb_state *s, int level, const uint8_t **stream0,
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* encoder-fused level errors (fp32 data)                                     */
+/* NegaBinaryBPEncoder::encode(..., level_errors) (NegaBinaryBPEncoder.hpp:   */
+/* 65-125) with collect_level_errors (:231-239), and GroupedBPEncoder::encode */
+/* (..., level_errors) (GroupedBPEncoder.hpp:76-145) with collect_level_errors*/
+/* (:324-335). Per element, nb + 1 squared truncation errors in the shifted   */
+/* units (value * 2^(nb - e), e + 2 for NegaBinary), each computed with the   */
+/* reference's types (float products where it multiplies floats, double       */
+/* otherwise); the table is scaled by 2^(2 (e - nb)) at the end. Both         */
+/* collectors take float / uint32 arguments, so only fp32 data is defined.    */
+/* ------------------------------------------------------------------------ */
+typedef void (*fused_emit_fn)(void *ctx, int bin, double term);
+
+static uint32_t fused_mask(int k) { return k >= 31 ? 0x7fffffffu : ((1u << k) - 1u); }
+
+static void nb_fused_terms(float x, int exp2, int nb, fused_emit_fn emit, void *ctx) {
+    const float s = ldexpf(x, nb - exp2);
+    const int32_t q = (int32_t)s;
+    const uint32_t nbq = ((uint32_t)q + 0xaaaaaaaau) ^ 0xaaaaaaaau;
+    const float m = s - (float)q;
+    const float mm = m * m;
+    emit(ctx, nb, (double)mm);
+    for (int k = 1; k < nb; k++) {
+        const int32_t v = (int32_t)(((nbq & fused_mask(k)) ^ 0xaaaaaaaau) - 0xaaaaaaaau);
+        const double d = (double)v + (double)m;
+        emit(ctx, nb - k, d * d);
+    }
+    const float ss = s * s;
+    emit(ctx, 0, (double)ss);
+}
+
+static void gb_fused_terms(float x, int exp, int nb, fused_emit_fn emit, void *ctx) {
+    const float s = ldexpf(x, nb - exp);
+    const float data = fabsf(s);
+    const uint32_t fp = (uint32_t)data;
+    const float mf = data - (float)fp;
+    const double m = (double)mf;
+    emit(ctx, nb, m * m);
+    for (int k = 1; k < nb; k++) {
+        const double d = (double)(fp & fused_mask(k)) + m;
+        emit(ctx, nb - k, d * d);
+    }
+    const float dd = data * data;
+    emit(ctx, 0, (double)dd);
+}
+
+static void fused_emit_seq(void *ctx, int bin, double term) { ((double *)ctx)[bin] += term; }
+
+typedef struct { u128 *bucket; } fused_exact_ctx;
+static void fused_emit_exact(void *ctx, int bin, double term) {
+    fused_exact_ctx *c = (fused_exact_ctx *)ctx;
+    uint64_t b;
+    memcpy(&b, &term, 8);
+    const unsigned be = (unsigned)((b >> 52) & 0x7ffu);
+    const uint64_t mant = b & ((1ull << 52) - 1);
+    const unsigned slot = be ? be : 1;
+    const uint64_t M = be ? (mant | (1ull << 52)) : mant;
+    c->bucket[(size_t)bin * 2048 + slot] += (u128)M;
+}
+
+typedef void (*fused_terms_fn)(float x, int e, int nb, fused_emit_fn emit, void *ctx);
+
+static void fused_level_errors(fused_terms_fn terms, const float *x, uint64_t n, int e, int nb, int exact,
+                               double *out) {
+    for (int i = 0; i <= nb; i++) out[i] = 0;
+    if (!exact) {
+        for (uint64_t i = 0; i < n; i++) terms(x[i], e, nb, fused_emit_seq, out);
+    } else {
+        fused_exact_ctx c;
+        c.bucket = (u128 *)calloc((size_t)(nb + 1) * 2048, sizeof(u128));
+        for (uint64_t i = 0; i < n; i++) terms(x[i], e, nb, fused_emit_exact, &c);
+        for (int bin = 0; bin <= nb; bin++) {
+            big_t big;
+            memset(&big, 0, sizeof(big));
+            for (unsigned slot = 1; slot < 2048; slot++) {
+                const u128 v = c.bucket[(size_t)bin * 2048 + slot];
+                if (v) big_add_shifted(&big, v, (int)slot - 1);
+            }
+            out[bin] = big_to_double(&big, -1074);
+        }
+        free(c.bucket);
+    }
+    for (int i = 0; i <= nb; i++) out[i] = ldexp(out[i], 2 * (e - nb));
+}
+
+void orc_nb_level_errors_f32(const float *x, uint64_t n, int32_t exp, int nb, double *out) {
+    fused_level_errors(nb_fused_terms, x, n, exp + 2, nb, 1, out);
+}
+void orc_nb_level_errors_seq_f32(const float *x, uint64_t n, int32_t exp, int nb, double *out) {
+    fused_level_errors(nb_fused_terms, x, n, exp + 2, nb, 0, out);
+}
+void orc_gb_level_errors_f32(const float *x, uint64_t n, int32_t exp, int nb, double *out) {
+    fused_level_errors(gb_fused_terms, x, n, exp, nb, 1, out);
+}
+void orc_gb_level_errors_seq_f32(const float *x, uint64_t n, int32_t exp, int nb, double *out) {
+    fused_level_errors(gb_fused_terms, x, n, exp, nb, 0, out);
+}

@@ -95,6 +95,13 @@ void orc_collect_squared_f32(const float *x, uint64_t n, int nb, float maxerr, d
 void orc_collect_squared_f64(const double *x, uint64_t n, int nb, double maxerr, double *out);
 void orc_collect_squared_seq_f32(const float *x, uint64_t n, int nb, float maxerr, double *out);
 void orc_collect_squared_seq_f64(const double *x, uint64_t n, int nb, double maxerr, double *out);
+/* encoder-fused level errors (NegaBinaryBPEncoder.hpp:65-125,231-239;
+ * GroupedBPEncoder.hpp:76-145,324-335), fp32 data, nb <= 32: exactly rounded
+ * sums of the reference's per-element terms; *_seq: reference order.      */
+void orc_nb_level_errors_f32(const float *x, uint64_t n, int32_t exp, int nb, double *out);
+void orc_nb_level_errors_seq_f32(const float *x, uint64_t n, int32_t exp, int nb, double *out);
+void orc_gb_level_errors_f32(const float *x, uint64_t n, int32_t exp, int nb, double *out);
+void orc_gb_level_errors_seq_f32(const float *x, uint64_t n, int32_t exp, int nb, double *out);
 
 /* ---- MGARD orthogonal decompose / recompose (in place, strided box) ----- */
 /* strides may be NULL (dense). Returns 0, or -1 for unsupported geometry.  */

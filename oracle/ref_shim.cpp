@@ -182,6 +182,25 @@ int ref_gb_decode_f64(void *h, int level, const uint8_t *const *streams, int32_t
     return gb_decode<MDR::GroupedBPEncoder<double, uint64_t>>(h, level, streams, n, exp, start, k, out);
 }
 
+// the encoders' level-error overloads (NegaBinaryBPEncoder.hpp:65-125,
+// GroupedBPEncoder.hpp:76-145); n >= the block size
+void ref_nb_level_errors_f32(const float *x, int32_t n, int32_t exp, int nb, double *out) {
+    MDR::NegaBinaryBPEncoder<float, uint32_t> enc;
+    std::vector<uint32_t> sizes;
+    std::vector<double> le;
+    auto streams = enc.encode(x, n, exp, (uint8_t)nb, sizes, le);
+    for (auto *p : streams) free(p);
+    std::copy(le.begin(), le.end(), out);
+}
+void ref_gb_level_errors_f32(const float *x, int32_t n, int32_t exp, int nb, double *out) {
+    MDR::GroupedBPEncoder<float, uint32_t> enc;
+    std::vector<uint32_t> sizes;
+    std::vector<double> le;
+    auto streams = enc.encode(x, n, exp, (uint8_t)nb, sizes, le);
+    for (auto *p : streams) free(p);
+    std::copy(le.begin(), le.end(), out);
+}
+
 void ref_collect_max_f32(float bound, int nb, double *out) {
     MDR::MaxErrorCollector<float> c;
     auto v = c.collect_level_error(nullptr, 0, nb, bound);

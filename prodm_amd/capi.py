@@ -78,6 +78,10 @@ _SIGS = {
     "prodm_interpret_retrieve_size": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
                                                 vp, vp, vp, C.c_double, vp, vp]),
     "prodm_max_estimator_ob_constant": (C.c_double, [C.c_int, C.c_int]),
+    "prodm_level_errors_workspace_bytes": (sz, [u64, C.c_int]),
+    "prodm_negabinary_level_errors_f32": (C.c_int, [vp, u64, i32, C.c_int, vp, vp, sz, vp]),
+    "prodm_grouped_level_errors_f32": (C.c_int, [vp, u64, i32, C.c_int, vp, vp, sz, vp]),
+    "prodm_mdr_level_errors": (C.c_int, [vp, C.c_int, vp, vp]),
     "prodm_mdr_batch_create": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int]),
     "prodm_mdr_batch_destroy": (C.c_int, [vp]),
     "prodm_mdr_batch_fields": (C.c_int, [vp]),

@@ -523,6 +523,16 @@ int prodm_collect_squared_f64(const double *d_x, uint64_t n, int nb, double maxe
     return collect_squared_impl<double>(d_x, n, nb, maxerr, h_out, d_ws, ws_bytes, (hipStream_t)stream);
 }
 
+size_t prodm_level_errors_workspace_bytes(uint64_t n, int nb) { return fused_errors_workspace_bytes(n, nb); }
+int prodm_negabinary_level_errors_f32(const float *d_x, uint64_t n, int32_t exp, int nb, double *h_out, void *d_ws,
+                                      size_t ws_bytes, prodm_stream_t stream) {
+    return fused_level_errors(0, d_x, n, exp, nb, h_out, d_ws, ws_bytes, (hipStream_t)stream);
+}
+int prodm_grouped_level_errors_f32(const float *d_x, uint64_t n, int32_t exp, int nb, double *h_out, void *d_ws,
+                                   size_t ws_bytes, prodm_stream_t stream) {
+    return fused_level_errors(1, d_x, n, exp, nb, h_out, d_ws, ws_bytes, (hipStream_t)stream);
+}
+
 // ---- plan ---------------------------------------------------------------------------
 int prodm_mdr_plan_create(prodm_mdr_plan **out, const uint32_t *dims, int ndims, int target_level, int nb, int dtype,
                           int encoder) {
@@ -615,6 +625,25 @@ int prodm_mdr_level_info(prodm_mdr_plan *p, int level, double *max_abs, int32_t 
     if (exp) *exp = e;
     if (elements) *elements = p->g.elems[level];
     return kOk;
+}
+
+int prodm_mdr_level_errors(prodm_mdr_plan *p, int level, double *h_out, prodm_stream_t stream) {
+    if (!p || level < 0 || level > p->g.L || !h_out) { set_error("level_errors: bad level"); return kErrArg; }
+    if (p->dtype != PRODM_F32) { set_error("level_errors: the encoders' level errors are defined for fp32 data"); return kErrUnsupported; }
+    if (p->nb > 32) { set_error("level_errors: at most 32 bitplanes"); return kErrUnsupported; }
+    double mx;
+    int32_t e;
+    uint64_t n;
+    if (int rc = prodm_mdr_level_info(p, level, &mx, &e, &n)) return rc;
+    const size_t need = fused_errors_workspace_bytes(n, p->nb);
+    void *ws = nullptr;
+    PRODM_HIP(hipMalloc(&ws, need));
+    const float *x = static_cast<const float *>(p->levels) + p->level_off[level];
+    const int rc = fused_level_errors(p->encoder == PRODM_ENC_NEGABINARY ? 0 : 1, x, n, e, p->nb, h_out, ws, need,
+                                      (hipStream_t)stream);
+    (void)hipStreamSynchronize((hipStream_t)stream);
+    (void)hipFree(ws);
+    return rc;
 }
 
 int prodm_mdr_plane(prodm_mdr_plan *p, int level, int plane, const void **d_ptr, uint64_t *bytes) {

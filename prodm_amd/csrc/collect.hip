@@ -282,4 +282,203 @@ template <class T> int finish_collect_squared(const void *h, int nb, T maxerr, d
 template int finish_collect_squared<float>(const void *, int, float, double *);
 template int finish_collect_squared<double>(const void *, int, double, double *);
 
+// ---- encoder-fused level errors ------------------------------------------------------
+// NegaBinaryBPEncoder::encode(..., level_errors) (NegaBinaryBPEncoder.hpp:65-125,
+// collect_level_errors :231-239) and GroupedBPEncoder::encode(..., level_errors)
+// (GroupedBPEncoder.hpp:76-145, :324-335), for fp32 data (the reference's
+// collectors take float / uint32). Per element, in the shifted units
+// s = x 2^(nb - e) (e + 2 for NegaBinary), bin nb - k holds the squared
+// value of the digits below the top nb - k planes plus the fraction m, bin
+// nb the fraction alone and bin 0 s^2; bins 0 and nb square in float as the
+// reference does. Each entry is the exactly rounded sum over elements, then
+// scaled by 2^(2 (e - nb)).
+//
+// Exactness: in shifted units every term of an element with |s| >= 1 is a
+// multiple of 2^-46 (the fraction's step is at least 2^-23) and every term
+// of an element with 2^-32 <= |s| < 1 a multiple of 2^-110, so terms are
+// integers in units of Q = 2^-110 below 2^176 (3 limbs, 2^13 terms per
+// workgroup). Terms of smaller elements go to a per-(bin, exponent)
+// histogram of 128-bit mantissa sums.
+namespace {
+constexpr int kFeLimbs = 3;
+constexpr int kFeLog2Q = -110;
+constexpr int kFeThreads = 256;
+constexpr int kFeEPT = 32;
+constexpr int kFeExpBuckets = 2048;
+
+__device__ __forceinline__ uint32_t fe_mask(int k) { return k >= 31 ? 0x7fffffffu : ((1u << k) - 1u); }
+
+template <int ENC>
+__device__ __forceinline__ double fe_term(float s, float m, uint32_t bits, int nb, int bin) {
+    if (ENC == 0) {  // NegaBinary: bits = negabinary digits of (int32) s
+        if (bin == nb) { const float mm = m * m; return (double)mm; }
+        if (bin == 0) { const float ss = s * s; return (double)ss; }
+        const int32_t v = (int32_t)(((bits & fe_mask(nb - bin)) ^ 0xaaaaaaaau) - 0xaaaaaaaau);
+        const double d = (double)v + (double)m;
+        return d * d;
+    }
+    // GroupedBP: s = |shifted|, bits = (uint32) s, m = s - bits (float)
+    const double md = (double)m;
+    if (bin == nb) return md * md;
+    if (bin == 0) { const float ss = s * s; return (double)ss; }
+    const double d = (double)(bits & fe_mask(nb - bin)) + md;
+    return d * d;
+}
+
+template <int ENC>
+__global__ __launch_bounds__(kFeThreads) void k_fused_errors(const float *__restrict__ x, uint64_t n, int shift,
+                                                             int nb, uint64_t *__restrict__ partial,
+                                                             unsigned long long *__restrict__ under) {
+    constexpr int LM = kFeLimbs;
+    __shared__ uint64_t wacc[kFeThreads / 64][33][LM];
+    const uint32_t lane = lane_id(), wv = threadIdx.x / 64;
+    const uint64_t base = (uint64_t)blockIdx.x * kFeThreads * kFeEPT;
+    float s[kFeEPT], m[kFeEPT];
+    uint32_t bits[kFeEPT];
+#pragma unroll
+    for (int i = 0; i < kFeEPT; i++) {
+        const uint64_t e = base + (uint64_t)i * kFeThreads + threadIdx.x;
+        const float v = e < n ? x[e] : 0.0f;
+        const float sh = ldexpf(v, shift);
+        if (ENC == 0) {
+            // (int32) shifted as in the reference (in range: |x| < 2^e)
+            const int32_t q = (sh == sh && fabsf(sh) < 2147483648.0f) ? (int32_t)sh : (int32_t)0x80000000;
+            bits[i] = ((uint32_t)q + 0xaaaaaaaau) ^ 0xaaaaaaaau;
+            s[i] = sh;
+            m[i] = sh - (float)q;
+        } else {
+            const float a = fabsf(sh);
+            const uint32_t fp = (a == a && a < 4294967296.0f) ? (uint32_t)a : 0u;
+            bits[i] = fp;
+            s[i] = a;
+            m[i] = a - (float)fp;
+        }
+    }
+    for (int bin = 0; bin <= nb; bin++) {
+        uint64_t a[LM];
+#pragma unroll
+        for (int j = 0; j < LM; j++) a[j] = 0;
+#pragma unroll 4
+        for (int i = 0; i < kFeEPT; i++) {
+            const double t = fe_term<ENC>(s[i], m[i], bits[i], nb, bin);
+            if (t == 0.0) continue;
+            uint64_t tb;
+            __builtin_memcpy(&tb, &t, 8);
+            const int be = (int)((tb >> 52) & 0x7ff);
+            const uint64_t M = (tb & ((1ull << 52) - 1)) | (be ? (1ull << 52) : 0);
+            const int E = (be ? be : 1) - 1075;  // t = M 2^E
+            const int sft = E - kFeLog2Q;
+            if (sft < 0) {
+                // 128-bit bucket: low word with carry into the high word
+                unsigned long long *bk = under + ((size_t)bin * kFeExpBuckets + (be ? be : 1)) * 2;
+                const unsigned long long old = atomicAdd(bk, (unsigned long long)M);
+                if (old + M < old) atomicAdd(bk + 1, 1ull);
+                continue;
+            }
+            const int li = sft >> 6, off = sft & 63;
+            const uint64_t plo = M << off;
+            const uint64_t phi = off ? (M >> (64 - off)) : 0;
+            uint64_t carry = 0;
+#pragma unroll
+            for (int j = 0; j < LM; j++) {
+                const uint64_t add = (j == li ? plo : 0) + (j == li + 1 ? phi : 0);
+                const uint64_t s1 = a[j] + add;
+                const uint64_t c1 = s1 < add;
+                const uint64_t s2 = s1 + carry;
+                const uint64_t c2 = s2 < carry;
+                a[j] = s2;
+                carry = c1 | c2;
+            }
+        }
+        for (int o = 32; o >= 1; o >>= 1) {
+            uint64_t b[LM];
+#pragma unroll
+            for (int j = 0; j < LM; j++) b[j] = __shfl_xor(a[j], o, 64);
+            uint64_t carry = 0;
+#pragma unroll
+            for (int j = 0; j < LM; j++) {
+                const uint64_t s1 = a[j] + b[j];
+                const uint64_t c1 = s1 < b[j];
+                const uint64_t s2 = s1 + carry;
+                const uint64_t c2 = s2 < carry;
+                a[j] = s2;
+                carry = c1 | c2;
+            }
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int j = 0; j < LM; j++) wacc[wv][bin][j] = a[j];
+        }
+    }
+    __syncthreads();
+    for (int bin = threadIdx.x; bin <= nb; bin += kFeThreads) {
+        uint64_t a[LM];
+#pragma unroll
+        for (int j = 0; j < LM; j++) a[j] = 0;
+        for (int w = 0; w < kFeThreads / 64; w++) {
+            uint64_t carry = 0;
+#pragma unroll
+            for (int j = 0; j < LM; j++) {
+                const uint64_t bj = wacc[w][bin][j];
+                const uint64_t s1 = a[j] + bj;
+                const uint64_t c1 = s1 < bj;
+                const uint64_t s2 = s1 + carry;
+                const uint64_t c2 = s2 < carry;
+                a[j] = s2;
+                carry = c1 | c2;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < LM; j++) partial[((uint64_t)blockIdx.x * (nb + 1) + bin) * LM + j] = a[j];
+    }
+}
+
+uint32_t fe_nwg(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, (n + kFeThreads * kFeEPT - 1) / (kFeThreads * kFeEPT)); }
+size_t fe_partial_bytes(uint64_t n, int nb) { return (size_t)fe_nwg(n) * (nb + 1) * kFeLimbs * 8; }
+size_t fe_under_bytes(int nb) { return (size_t)(nb + 1) * kFeExpBuckets * 16; }
+}  // namespace
+
+size_t fused_errors_workspace_bytes(uint64_t n, int nb) {
+    return fe_partial_bytes(n, nb) + fe_under_bytes(nb);
+}
+
+int fused_level_errors(int enc, const float *x, uint64_t n, int32_t exp, int nb, double *out, void *ws,
+                       size_t ws_bytes, hipStream_t s) {
+    if (nb < 1 || nb > 32 || !out || (enc != 0 && enc != 1)) { set_error("level errors: bad arguments"); return kErrArg; }
+    if (n > 0 && !x) { set_error("level errors: null input"); return kErrArg; }
+    const size_t pb = fe_partial_bytes(n, nb), ub = fe_under_bytes(nb);
+    if (ws_bytes < pb + ub || !ws) { set_error("level errors: workspace too small"); return kErrArg; }
+    const int e = enc == 0 ? exp + 2 : exp;  // NegaBinaryBPEncoder.hpp:68
+    uint64_t *partial = static_cast<uint64_t *>(ws);
+    unsigned long long *under = reinterpret_cast<unsigned long long *>(static_cast<char *>(ws) + pb);
+    const uint32_t nwg = fe_nwg(n);
+    int rc = launch_fill_u64(reinterpret_cast<uint64_t *>(under), ub / 8, 0ull, s);
+    if (!rc && n == 0) rc = launch_fill_u64(partial, pb / 8, 0ull, s);
+    if (rc) return rc;
+    if (n) {
+        if (enc == 0)
+            hipLaunchKernelGGL(k_fused_errors<0>, dim3(nwg), dim3(kFeThreads), 0, s, x, n, nb - e, nb, partial, under);
+        else
+            hipLaunchKernelGGL(k_fused_errors<1>, dim3(nwg), dim3(kFeThreads), 0, s, x, n, nb - e, nb, partial, under);
+        PRODM_CHECK_LAUNCH();
+    }
+    std::vector<uint64_t> host((pb + ub) / 8);
+    PRODM_HIP(hipMemcpyAsync(host.data(), ws, pb + ub, hipMemcpyDeviceToHost, s));
+    PRODM_HIP(hipStreamSynchronize(s));
+    const uint64_t *hp = host.data(), *hu = host.data() + pb / 8;
+    const int wshift = kFeLog2Q - (-1074);
+    const size_t limbs = (size_t)(2048 + 128 + 64 * kFeLimbs + 64) / 64 + 2;
+    for (int k = 0; k <= nb; k++) {
+        Big b(limbs);
+        for (uint32_t g = 0; g < nwg; g++)
+            for (int j = 0; j < kFeLimbs; j++) b.add_shifted(hp[((uint64_t)g * (nb + 1) + k) * kFeLimbs + j], wshift + 64 * j);
+        for (int slot = 1; slot < kFeExpBuckets; slot++) {
+            b.add_shifted(hu[((size_t)k * kFeExpBuckets + slot) * 2], slot - 1);
+            b.add_shifted(hu[((size_t)k * kFeExpBuckets + slot) * 2 + 1], slot - 1 + 64);
+        }
+        out[k] = std::ldexp(b.to_double(-1074), 2 * (e - nb));
+    }
+    return kOk;
+}
+
 }  // namespace prodm

@@ -86,6 +86,13 @@ int launch_collect_squared(const T *x, uint64_t n, int nb, T max_level_error, vo
 template <class T> int finish_collect_squared(const void *h_ws_copy, int nb, T max_level_error, double *out);
 template <class T> size_t sq_result_bytes(int nb);
 
+// ---- encoder-fused level errors (NegaBinaryBPEncoder.hpp:65-125,231-239,
+// GroupedBPEncoder.hpp:76-145,324-335), fp32: enc 0 NegaBinary, 1 GroupedBP.
+// Exactly rounded table of nb + 1 doubles on the host; synchronizes.
+size_t fused_errors_workspace_bytes(uint64_t n, int nb);
+int fused_level_errors(int enc, const float *x, uint64_t n, int32_t exp, int nb, double *out, void *ws,
+                       size_t ws_bytes, hipStream_t s);
+
 // ---- MGARD ------------------------------------------------------------------
 size_t mgard_workspace_bytes(const LevelGeom &g, size_t elem);
 // Coefficient addressing of a decompose/recompose: full array (reordered

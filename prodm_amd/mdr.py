@@ -224,6 +224,19 @@ def collect_squared(x, nb, maxerr):
     return np.array(out[:])
 
 
+def level_errors(x, exp, nb, encoder=capi.NEGABINARY):
+    """Level-error table of the encoders' level_errors overloads (fp32):
+    entry j = squared error of keeping the first j planes of the codec."""
+    if x.dtype != torch.float32:
+        raise ValueError("the encoders' level errors are defined for float32 data")
+    n = x.numel()
+    ws = _ws(lib().prodm_level_errors_workspace_bytes(n, nb))
+    out = (C.c_double * (nb + 1))()
+    fn = lib().prodm_negabinary_level_errors_f32 if encoder == capi.NEGABINARY else lib().prodm_grouped_level_errors_f32
+    check(fn(_ptr(x), n, int(exp), nb, out, _ptr(ws), ws.numel(), _stream()), "level errors")
+    return np.array(out[:])
+
+
 # ---- pipeline plan ---------------------------------------------------------------------------
 class Plan:
     """Device-resident refactor/retrieve of one field (prodm_mdr_plan)."""
@@ -248,6 +261,12 @@ class Plan:
         m, e, n = C.c_double(), C.c_int32(), C.c_uint64()
         check(lib().prodm_mdr_level_info(self.h, level, C.byref(m), C.byref(e), C.byref(n)), "level info")
         return m.value, e.value, n.value
+
+    def level_errors(self, level):
+        """the plan encoder's level-error table of a level (fp32 plans)"""
+        out = (C.c_double * (self.nb + 1))()
+        check(lib().prodm_mdr_level_errors(self.h, level, out, _stream()), "plan level errors")
+        return np.array(out[:])
 
     def plane(self, level, p):
         ptr, nbytes = C.c_void_p(), C.c_uint64()

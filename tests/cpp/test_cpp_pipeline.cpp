@@ -16,6 +16,8 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -234,6 +236,29 @@ static void component_checks(const std::vector<float> &u, int nb, Encoder enc) {
     for (int i = 0; i < n; i++) worst = std::max(worst, std::fabs(acc[i] - one[i]));
     EXPECT(worst <= std::ldexp(1.0, e - nb + 3), "progressive deltas differ from one-shot decode by %.3e", worst);
     std::free(one);
+    // the level-error overload: the same streams, and entry j = squared error
+    // of decoding the first j planes (float-squared bins 0 and nb: 2^-22)
+    std::vector<uint32_t> sizes2;
+    std::vector<double> le;
+    auto streams2 = enc.encode(u.data(), n, e, nb, sizes2, le);
+    EXPECT(sizes2 == sizes && (int)le.size() == nb + 1, "level-error encode changed the streams");
+    for (int p = 0; p < nb && sizes2 == sizes; p++)
+        EXPECT(std::memcmp(streams[p], streams2[p], sizes[p]) == 0, "level-error encode changed plane %d", p);
+    for (int j : {0, 1, nb / 2, nb}) {
+        double want = 0;
+        if (j == 0) {
+            for (int i = 0; i < n; i++) want += (double)u[i] * u[i];
+        } else {
+            std::vector<const uint8_t *> part(cs.begin(), cs.begin() + j);
+            // a fresh level key per prefix: GroupedBP decode state is progressive per level
+            float *d = enc.progressive_decode(part, n, e, 0, j, 10 + j);
+            for (int i = 0; i < n; i++) want += ((double)u[i] - d[i]) * ((double)u[i] - d[i]);
+            std::free(d);
+        }
+        const double tol = (j == 0 || j == nb) ? std::ldexp(1.0, -22) : 1e-9;
+        EXPECT(std::fabs(le[j] - want) <= tol * want + 1e-300, "level error %d: %.6e vs %.6e", j, le[j], want);
+    }
+    for (auto *s : streams2) std::free(s);
     for (auto *s : streams) std::free(s);
 }
 

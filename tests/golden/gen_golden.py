@@ -151,6 +151,19 @@ def gen_collectors(out):
         out[f"sq{i}_nb"] = np.array(nb)
         out[f"sq{i}"] = ob.collect_squared(x, nb, m, mode="ref")
     out["sq_count"] = np.array(3)
+    # the encoders' level_errors overloads (fp32): NegaBinary and GroupedBP
+    fle = ((1000, 32, False), (777, 16, False), (64, 30, False), (4096, 32, True))
+    for i, (n, nb, tiny) in enumerate(fle):
+        x = codec_input(n, np.float32, 400 + i)
+        if tiny:  # a wide dynamic range: elements far below 2^-nb of the maximum
+            x[: n // 4] *= np.float32(2.0 ** -40)
+        e = level_exp(x)
+        out[f"fle{i}_x"] = x
+        out[f"fle{i}_exp"] = np.array(e)
+        out[f"fle{i}_nb"] = np.array(nb)
+        out[f"fle{i}_negabinary"] = ob.fused_level_errors(x, e, nb, "nb", mode="ref")
+        out[f"fle{i}_grouped"] = ob.fused_level_errors(x, e, nb, "gb", mode="ref")
+    out["fle_count"] = np.array(len(fle))
     out["maxest_c"] = np.array([ob.ref_max_estimator_c(nd) for nd in (1, 2, 3)])
 
 

@@ -246,6 +246,23 @@ def collect_squared(x, nb, maxerr, mode="exact"):
     return out
 
 
+def fused_level_errors(x, exp, nb, enc, mode="exact"):
+    """Level-error table of the encoders' level_errors overloads (fp32):
+    enc "nb" (NegaBinaryBPEncoder) or "gb" (GroupedBPEncoder); mode "exact"
+    (exactly rounded sums), "seq" (reference order) or "ref" (the compiled
+    reference, n >= 32)."""
+    assert x.dtype == np.float32
+    out = np.zeros(nb + 1, np.float64)
+    if mode == "ref":
+        fn = getattr(ref(), f"ref_{enc}_level_errors_f32")
+        fn.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int, C.c_void_p]
+    else:
+        fn = getattr(oracle(), f"orc_{enc}_level_errors{'_seq' if mode == 'seq' else ''}_f32")
+        fn.argtypes = [C.c_void_p, C.c_uint64, C.c_int32, C.c_int, C.c_void_p]
+    fn(x.ctypes.data_as(C.c_void_p), x.size, int(exp), int(nb), out.ctypes.data_as(C.c_void_p))
+    return out
+
+
 # ---- MGARD ----------------------------------------------------------------------------
 def decompose(data, dims, L, strides=None, hierarchical=False):
     t = _tp(data.dtype)

@@ -103,6 +103,25 @@ def test_collect_squared_exact_vs_sequential(g):
         assert np.all(np.abs(ex - seq) <= x.size * 2.0 ** -52 * np.abs(seq) + 1e-300)
 
 
+def test_fused_level_errors_sequential_order(g):
+    """The oracle's reference-order sums reproduce the encoders' level_errors
+    overloads (NegaBinaryBPEncoder.hpp:65-125, GroupedBPEncoder.hpp:76-145)
+    bit for bit, including a case with elements far below 2^-nb of the max."""
+    for i in range(int(g["fle_count"])):
+        x, e, nb = g[f"fle{i}_x"], int(g[f"fle{i}_exp"]), int(g[f"fle{i}_nb"])
+        for enc, key in (("nb", "negabinary"), ("gb", "grouped")):
+            np.testing.assert_array_equal(ob.fused_level_errors(x, e, nb, enc, mode="seq"), g[f"fle{i}_{key}"])
+
+
+def test_fused_level_errors_exact_vs_sequential(g):
+    for i in range(int(g["fle_count"])):
+        x, e, nb = g[f"fle{i}_x"], int(g[f"fle{i}_exp"]), int(g[f"fle{i}_nb"])
+        for enc, key in (("nb", "negabinary"), ("gb", "grouped")):
+            ex = ob.fused_level_errors(x, e, nb, enc, mode="exact")
+            seq = g[f"fle{i}_{key}"]
+            assert np.all(np.abs(ex - seq) <= x.size * 2.0 ** -52 * np.abs(seq) + 1e-300), (i, enc)
+
+
 def test_max_estimator_constant(g):
     from prodm_amd import interpret
     np.testing.assert_allclose([interpret.max_estimator_ob_c(nd) for nd in (1, 2, 3)], g["maxest_c"], rtol=0,

@@ -348,6 +348,44 @@ def test_squared_collector_is_exactly_rounded(nb):
     np.testing.assert_array_equal(got, np.array(want))
 
 
+# ------------------------------------------------ encoder-fused level errors ---
+
+
+@pytest.mark.parametrize("enc,nb", [("nb", 32), ("gb", 32), ("nb", 12), ("gb", 18)])
+def test_fused_level_errors_exactly_rounded(enc, nb):
+    """exact mode = the correctly rounded sum of the per-element tables"""
+    x = codec_x(200, np.float32, 21)
+    e = ob.frexp_exp(ob.maxabs(x), np.float32)
+    terms = [ob.fused_level_errors(x[i:i + 1], e, nb, enc, mode="seq") for i in range(x.size)]
+    # per-element tables are already scaled by a power of two: the sum of the
+    # scaled terms rounds like the scaled sum of the terms
+    want = [float(sum((Fraction(float(t[k])) for t in terms), Fraction(0))) for k in range(nb + 1)]
+    np.testing.assert_array_equal(ob.fused_level_errors(x, e, nb, enc, mode="exact"), np.array(want))
+
+
+@pytest.mark.parametrize("enc,nb", [("nb", 32), ("nb", 16), ("gb", 32), ("gb", 20)])
+def test_fused_level_errors_are_plane_prefix_errors(enc, nb):
+    """Entry j is the squared error of keeping the first j planes: the sum of
+    (x - decode_j(x))^2 with the codec's own decoder (j = 0: sum x^2; j = nb:
+    the residual below the last plane). Bins 0 and nb square in float as the
+    reference does (relative 2^-23); the others are double."""
+    x = codec_x(512, np.float32, 5 + nb)
+    e = ob.frexp_exp(ob.maxabs(x), np.float32)
+    got = ob.fused_level_errors(x, e, nb, enc, mode="exact")
+    if enc == "nb":
+        planes = ob.nb_encode(x, e, nb)
+        dec = lambda j: ob.nb_decode(planes[:j], x.size, e, 0, j, np.float32)
+    else:
+        streams = ob.gb_encode(x, e, nb)
+        dec = lambda j: ob.GBDecoder(np.float32).decode(0, streams[:j], x.size, e, 0, j)
+    for j in range(nb + 1):
+        xt = np.zeros_like(x) if j == 0 else dec(j)
+        d = x.astype(np.float64) - xt.astype(np.float64)
+        want = float(sum((Fraction(float(v)) ** 2 for v in d), Fraction(0)))
+        tol = 2.0 ** -22 if j in (0, nb) else 1e-12
+        assert abs(got[j] - want) <= tol * want + 1e-300, (j, got[j], want)
+
+
 # ------------------------------------------------------- hierarchical basis ---
 
 

@@ -91,6 +91,15 @@ def test_grouped(dt, nb, n, scale):
         np.testing.assert_array_equal(r1.view(np.uint8), r2.view(np.uint8))
 
 
+@pytest.mark.parametrize("n,nb", [(2048, 32), (999, 24), (32, 2), (5000, 31)])
+@pytest.mark.parametrize("enc", ["nb", "gb"])
+def test_fused_level_errors_sequential(n, nb, enc):
+    x = x_of(n, np.float32, n + nb)
+    e = ob.frexp_exp(ob.maxabs(x), np.float32)
+    np.testing.assert_array_equal(ob.fused_level_errors(x, e, nb, enc, mode="seq"),
+                                  ob.fused_level_errors(x, e, nb, enc, mode="ref"))
+
+
 @pytest.mark.parametrize("n,nb", [(1000, 32), (777, 16), (64, 30)])
 def test_squared_sequential(n, nb):
     x = x_of(n, np.float32, n)
