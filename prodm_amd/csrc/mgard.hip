@@ -107,7 +107,10 @@ __device__ inline DimTile dim_tile(uint32_t tile, int TJ, uint32_t c, uint32_t n
 // Per fine plane: W (in LDS), coefficient / coarse-value / max stores, the
 // load stencil along dim 2 (LDS, FP64) and dim 1 (registers, one coarse
 // column per thread), and the dim-0 stencil as a running sum over planes.
-constexpr int kM1 = 8, kM2 = 32;
+// 7 x 33 coarse nodes: coarse extents are 2^k + 1, so 32-wide tiles would
+// leave a one-column tile at the end of every dim (257 = 8 * 32 + 1); the
+// 17 x 69 region still fits five elements per thread.
+constexpr int kM1 = 7, kM2 = 33;
 constexpr int kN1 = 2 * kM1 + 3, kN2 = 2 * kM2 + 3, kPL = kN1 * kN2;
 constexpr int kPER = (kPL + kLoadThreads - 1) / kLoadThreads;   // plane elements per thread
 constexpr int kG2 = kN1 * kM2;
@@ -163,8 +166,8 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt) {
     using GChar = PRODM_GLOBAL char;
     constexpr bool kDec = MODE == kDecompose;
     constexpr int kPE = kDec ? kPL : 1;  // input planes staged only by decompose
-    __shared__ T Ea[kPE], Eb[kPE], O[kPE], Wb[kPL];
-    __shared__ double G2[kG2];
+    __shared__ T E[kPE], O[kPE], Wb[kPL], Wb2[kPE];
+    __shared__ double G2[kG2], G2b[kDec ? kG2 : 1];
     const int tid = threadIdx.x;
     const uint32_t n0 = P.n[0], n1 = P.n[1], n2 = P.n[2], c0 = P.c[0], c1 = P.c[1], c2 = P.c[2];
     const MarchTile M = march_tile(bt, n0, n1, n2, c0, c1, c2, P.march);
@@ -263,11 +266,25 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt) {
     };
     const double w0_2 = centre(M.js2 + cj2, c2), w0_1 = centre(j1, c1);
 
-    // W of plane i0 into Wb from planes A (i0-1), B (i0+1), C (i0) [decompose]
-    auto make_w = [&](int i0, const T *A, const T *Bp, const T *C) __attribute__((always_inline)) {
-        const bool o0 = act0 && (i0 & 1);
-        const bool own0 = !act0 || (i0 >= 2 * M.js0 - 1 && i0 < 2 * (M.js0 + M.tj0) - 1);
-        const T *PA = o0 ? A : C, *PB = o0 ? Bp : C;
+    // W of the fine planes [decompose]. The multilinear interpolant of a node
+    // is the tree sum of its cell's 8 corners (an even dim repeating the same
+    // corner) times 1/8. Per plane the 4-corner part is
+    //   S = (P[q] + P[q + d2]) + (P[q + d1] + P[q + d1 + d2]),
+    // so on an even plane (both dim-0 corners in the plane itself) the tree
+    // is S + S = 2 S exactly and I = S / 4, and on an odd plane I = (S of the
+    // plane below + S of the plane above) / 8: the same operations in the
+    // same order, with each even plane's S computed once and kept per slot.
+    auto own_plane = [&](int i0) __attribute__((always_inline)) -> bool {
+        return !act0 || (i0 >= 2 * M.js0 - 1 && i0 < 2 * (M.js0 + M.tj0) - 1);
+    };
+    auto keep_max = [&](bool st, T w) __attribute__((always_inline)) {
+        const T aw = (st && w == w) ? fabs(w) : (T)0;
+        lmax = aw > lmax ? aw : lmax;
+    };
+    // even plane i0 staged in Ev: S per slot into s, W into Wout, coarse
+    // values into u_c and coefficients into the level buffer
+    auto even_plane = [&](int i0, const T *Ev, T *Wout, T (&s)[kPER]) __attribute__((always_inline)) {
+        const bool own0 = own_plane(i0);
         GChar *cp = (GChar *)((GT *)(ILV ? P.coef_ilv : P.coef_full) + coef_base(i0));
         GChar *up = (GChar *)((GT *)P.uc + (uint64_t)(i0 >> 1) * c1 * c2);
 #pragma unroll
@@ -277,37 +294,49 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt) {
             const uint32_t b = bits >> (4 * m);
             const int d1 = (b & 2) ? 2 * kN2 : 0, d2 = (b & 4) ? 2 : 0;
             const int q = qb[m];
-            // branch-free multilinear interpolant: the 8 corners of the cell,
-            // an even dim repeating the same value; duplicates double exactly,
-            // so this equals the per-case tree sum of the distinct corners
-            const T I = (((PA[q] + PA[q + d2]) + (PA[q + d1] + PA[q + d1 + d2])) +
-                         ((PB[q] + PB[q + d2]) + (PB[q + d1] + PB[q + d1 + d2]))) *
-                        (T)0.125;
-            const T cv = C[idx];
-            const bool coarse = !o0 && !(b & 6);
-            const T w = ((b & 1) && !coarse) ? cv - I : (T)0;
-            Wb[idx] = w;
+            const T S = (Ev[q] + Ev[q + d2]) + (Ev[q + d1] + Ev[q + d1 + d2]);
+            s[m] = S;
+            const T cv = Ev[idx];
+            const bool coarse = !(b & 6);
+            const T w = ((b & 1) && !coarse) ? cv - S * (T)0.25 : (T)0;
+            Wout[idx] = w;
             // one predicated store: the coarse value into u_c or the
             // coefficient into the level buffer
             const bool st = (b & 1) && (b & 8) && own0;
-            const uint32_t oa = offA[m], ob = offB[m];
-            const uint32_t oc = o0 ? ob : oa;
-            GChar *dst = coarse ? up + oa : cp + oc;
+            const uint32_t oa = offA[m];
+            GChar *dst = coarse ? up + oa : cp + oa;
             if (st) *(GT *)dst = coarse ? cv : w;
-            const T aw = (st && !coarse && w == w) ? fabs(w) : (T)0;
-            lmax = aw > lmax ? aw : lmax;
+            keep_max(st && !coarse, w);
+        }
+    };
+    // odd plane i0 staged in Ov, between even planes with sums sa and sb
+    auto odd_plane = [&](int i0, const T *Ov, T *Wout, const T (&sa)[kPER], const T (&sb)[kPER])
+        __attribute__((always_inline)) {
+        const bool own0 = own_plane(i0);
+        GChar *cp = (GChar *)((GT *)(ILV ? P.coef_ilv : P.coef_full) + coef_base(i0));
+#pragma unroll
+        for (int m = 0; m < kPER; m++) {
+            const int idx = tid + kLoadThreads * m;
+            if (kLoadThreads * (m + 1) > kPL && idx >= kPL) continue;
+            const uint32_t b = bits >> (4 * m);
+            const T I = (sa[m] + sb[m]) * (T)0.125;
+            const T w = (b & 1) ? Ov[idx] - I : (T)0;
+            Wout[idx] = w;
+            const bool st = (b & 1) && (b & 8) && own0;
+            if (st) *(GT *)(cp + offB[m]) = w;
+            keep_max(st, w);
         }
     };
     // load stencil along dim 2 from Wb into G2[r1][jj2]; this thread's jj2
     // is fixed (tid % kM2), its rows step by kLoadThreads / kM2
-    auto stencil2 = [&]() __attribute__((always_inline)) {
-        if (cj2 >= M.tj2) return;
+    auto stencil2 = [&](const T *Wsrc, double *G) __attribute__((always_inline)) {
         constexpr int kRowStep = kLoadThreads / kM2;
+        if (cj2 >= M.tj2 || cj1 >= kRowStep) return;
 #pragma unroll
         for (int m = 0; m < kG2PER; m++) {
             const int r1 = cj1 + kRowStep * m;
             if (kLoadThreads * (m + 1) > kG2 && r1 >= kN1) continue;
-            const T *row = Wb + r1 * kN2;
+            const T *row = Wsrc + r1 * kN2;
             double a;
             if (!act2) {
                 a = (double)row[0];
@@ -319,7 +348,7 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt) {
                 a = fma(kW1, (double)t[3], a);
                 a = fma(kW2, (double)t[4], a);
             }
-            G2[r1 * kM2 + cj2] = a;
+            G[r1 * kM2 + cj2] = a;
         }
     };
     // dim 1 for this thread's column (rows 2 cj1 .. 2 cj1 + 4 of G2), then
@@ -327,9 +356,9 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt) {
     // with |2 j0 - i0| <= 2, in increasing i0. Registers: ra = coarse plane
     // m-1, rb = m, rc = m+1 for the current even plane 2m; after it, m-1 is
     // complete and the ring rotates.
-    auto dim1 = [&]() __attribute__((always_inline)) -> double {
-        if (!act1) return G2[cj2];
-        const double *g = G2 + 2 * cj1 * kM2 + cj2;
+    auto dim1 = [&](const double *G) __attribute__((always_inline)) -> double {
+        if (!act1) return G[cj2];
+        const double *g = G + 2 * cj1 * kM2 + cj2;
         double h = kW2 * g[0];
         h = fma(kW1, g[kM2], h);
         h = fma(w0_1, g[2 * kM2], h);
@@ -346,9 +375,9 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt) {
     auto store_f = [&](int j0, double v) __attribute__((always_inline)) {
         ((PRODM_GLOBAL double *)P.f)[((uint64_t)j0 * c1 + j1) * P.fp + j2] = v;
     };
-    auto column = [&](int i0) __attribute__((always_inline)) {
+    auto column = [&](int i0, const double *G) __attribute__((always_inline)) {
         if (!has_col) return;
-        const double h = dim1();
+        const double h = dim1(G);
         if (!act0) {
             store_f(0, h);
             return;
@@ -375,45 +404,41 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt) {
     const int i0hi = act0 ? min((int)n0 - 1, 2 * (M.js0 + M.tj0 - 1) + 2) : 0;
     T pa[kPER], pb[kPER];
     if constexpr (kDec) {
-        // planes odd i0 and even i0 + 1 per step; E0 holds plane i0 - 1
-        auto step = [&](int i0, T *E0, T *E1) __attribute__((always_inline)) {
-            put(O, pa);
-            put(E1, pb);
-            if (i0 + 2 <= i0hi) { fetch(pa, i0 + 2); fetch(pb, i0 + 3); }
-            __syncthreads();
-            make_w(i0, E0, E1, O);
-            __syncthreads();
-            if (!P.hb) {
-                stencil2();
-                __syncthreads();
-                column(i0);
-            }
-            make_w(i0 + 1, E1, E1, E1);
-            __syncthreads();
-            if (!P.hb) {
-                stencil2();
-                __syncthreads();
-                column(i0 + 1);
-            }
-        };
+        // plane i0lo (even), then per step the odd plane i0 and the even
+        // plane i0 + 1: both W planes are built between two barriers, their
+        // dim-2 stencils into G2 / G2b between the next two, then the
+        // columns take them in plane order
+        T sprev[kPER];
         fetch(pa, i0lo);
-        put(Ea, pa);
+        put(E, pa);
         if (i0lo + 1 <= i0hi) { fetch(pa, i0lo + 1); fetch(pb, i0lo + 2); }
         __syncthreads();
-        make_w(i0lo, Ea, Ea, Ea);
+        even_plane(i0lo, E, Wb, sprev);
         __syncthreads();
         if (!P.hb) {
-            stencil2();
+            stencil2(Wb, G2);
             __syncthreads();
-            column(i0lo);
+            column(i0lo, G2);
         }
-        T *E0 = Ea, *E1 = Eb;
 #pragma unroll 1
         for (int i0 = i0lo + 1; i0 <= i0hi; i0 += 2) {
-            step(i0, E0, E1);
-            T *t = E0;
-            E0 = E1;
-            E1 = t;
+            put(O, pa);
+            put(E, pb);
+            if (i0 + 2 <= i0hi) { fetch(pa, i0 + 2); fetch(pb, i0 + 3); }
+            __syncthreads();
+            T scur[kPER];
+            even_plane(i0 + 1, E, Wb2, scur);
+            odd_plane(i0, O, Wb, sprev, scur);
+#pragma unroll
+            for (int m = 0; m < kPER; m++) sprev[m] = scur[m];
+            __syncthreads();
+            if (!P.hb) {
+                stencil2(Wb, G2);
+                stencil2(Wb2, G2b);
+                __syncthreads();
+                column(i0, G2);
+                column(i0 + 1, G2b);
+            }
         }
         if (P.key) {
             K mk = abs_key(lmax);
@@ -430,9 +455,9 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt) {
             put(Wb, pa);
             if (i0 + 1 <= i0hi) fetch(pa, i0 + 1);
             __syncthreads();
-            stencil2();
+            stencil2(Wb, G2);
             __syncthreads();
-            column(i0);
+            column(i0, G2);
         }
     }
 }
@@ -772,7 +797,7 @@ template <class T> struct Level0Entry {
 };
 
 template <class T>
-__global__ __launch_bounds__(kLoadThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_load_grouped(const LoadParams<T> *__restrict__ ps,
+__global__ __launch_bounds__(kLoadThreads) void k_load_grouped(const LoadParams<T> *__restrict__ ps,
                                                                const uint32_t *__restrict__ first, uint32_t nent,
                                                                uint32_t total) {
     for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
