@@ -481,15 +481,26 @@ static void thomas_factors(uint32_t m, double *inv, double *cp) {
 
 enum Epilogue : int { kEpiNone = 0, kEpiAdd = 1, kEpiSub = 2 };
 
+// The factors of interior nodes depend only on the node index (the end
+// diagonal 2 enters at node 0 and at the last node) and reach a bitwise
+// fixed point after a few nodes: the kernels read the table below nfix and
+// use lim for the interior nodes above it and last for node m - 1, the same
+// values the table holds. cp[i] = inv[i] (the off-diagonal is 1).
 template <class T> struct SolveParams {
     double *f;            // c0 x c1 rows of pitch fp
     uint32_t fp;
     uint32_t c[3];
-    const double *inv, *cp;
+    const double *inv;    // inv[0..m) of the solve dim
+    double lim, last;     // inv[i] for nfix <= i < m - 1, inv[m - 1]
+    uint32_t nfix;
     int epi;
     const T *uc_in;       // epilogue input (dense c)
     T *uc_out;            // epilogue output (dense c)
 };
+template <class T, class P>
+__device__ __forceinline__ double thomas_factor(const SolveParams<T> &S, P inv, uint32_t m, uint32_t i) {
+    return i < S.nfix ? inv[i] : (i == m - 1 ? S.last : S.lim);
+}
 
 // fidx: position in f (padded rows), uidx: position in the dense u_c
 template <class T> __device__ inline void epilogue(const SolveParams<T> &S, uint64_t fidx, uint64_t uidx, double z) {
@@ -508,7 +519,7 @@ __device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint
     const uint64_t line = line0 + wv;
     if (line >= nlines) return;
     const PRODM_GLOBAL double *x = (const PRODM_GLOBAL double *)S.f + line * S.fp;
-    const PRODM_GLOBAL double *inv = (const PRODM_GLOBAL double *)S.inv, *cpv = (const PRODM_GLOBAL double *)S.cp;
+    const PRODM_GLOBAL double *inv = (const PRODM_GLOBAL double *)S.inv;
     const uint32_t seg = (m + 63) / 64;
     const uint32_t b = lane * seg;
     const uint32_t e = min(m, b + seg);
@@ -520,7 +531,7 @@ __device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint
 #pragma unroll
     for (int i = 0; i < SEG; i++) {
         if (b + i < e) {
-            const double iv = inv[b + i];
+            const double iv = thomas_factor(S, inv, m, b + i);
             A = (v[i] - A) * iv;
             Bm = -Bm * iv;
         }
@@ -534,7 +545,7 @@ __device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint
 #pragma unroll
     for (int i = 0; i < SEG; i++) {
         if (b + i < e) {
-            y = (v[i] - y) * inv[b + i];
+            y = (v[i] - y) * thomas_factor(S, inv, m, b + i);
             v[i] = y;
         }
     }
@@ -543,7 +554,7 @@ __device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint
 #pragma unroll
     for (int i = SEG - 1; i >= 0; i--) {
         if (b + i < e) {
-            const double c = (b + i == m - 1) ? 0.0 : cpv[b + i];
+            const double c = (b + i == m - 1) ? 0.0 : thomas_factor(S, inv, m, b + i);
             C = v[i] - c * C;
             Dm = -c * Dm;
         }
@@ -557,7 +568,7 @@ __device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint
 #pragma unroll
     for (int i = SEG - 1; i >= 0; i--) {
         if (b + i < e) {
-            const double c = (b + i == m - 1) ? 0.0 : cpv[b + i];
+            const double c = (b + i == m - 1) ? 0.0 : thomas_factor(S, inv, m, b + i);
             z = v[i] - c * z;
             v[i] = z;
         }
@@ -599,10 +610,10 @@ __device__ __forceinline__ void solve_strided_lines(const SolveParams<T> &S, uin
         ustride = (uint64_t)c1 * c2;
         if (act) { base = r * fp + j2; ubase = r * c2 + j2; }
     }
-    // the factors are read-only while a solve runs: constant-space pointers
-    // let their wave-uniform loads go through the scalar cache
+    // the factors are read-only while a solve runs: a constant-space pointer
+    // lets their wave-uniform loads go through the scalar cache
     const PRODM_GLOBAL double *fg = (const PRODM_GLOBAL double *)S.f;
-    const PRODM_CONST double *inv = (const PRODM_CONST double *)S.inv, *cpv = (const PRODM_CONST double *)S.cp;
+    const PRODM_CONST double *inv = (const PRODM_CONST double *)S.inv;
     const uint32_t seg = (m + nw - 1) / nw;
     // wave-uniform segment bounds: the factor loads become scalar loads
     const uint32_t b = (uint32_t)__builtin_amdgcn_readfirstlane((int)(wv * seg));
@@ -614,7 +625,7 @@ __device__ __forceinline__ void solve_strided_lines(const SolveParams<T> &S, uin
 #pragma unroll
     for (int i = 0; i < SEG; i++) {
         if (b + i < e) {
-            const double iv = inv[b + i];
+            const double iv = thomas_factor(S, inv, m, b + i);
             A = (v[i] - A) * iv;
             Bm = -Bm * iv;
         }
@@ -628,7 +639,7 @@ __device__ __forceinline__ void solve_strided_lines(const SolveParams<T> &S, uin
 #pragma unroll
     for (int i = 0; i < SEG; i++) {
         if (b + i < e) {
-            y = (v[i] - y) * inv[b + i];
+            y = (v[i] - y) * thomas_factor(S, inv, m, b + i);
             v[i] = y;
         }
     }
@@ -636,7 +647,7 @@ __device__ __forceinline__ void solve_strided_lines(const SolveParams<T> &S, uin
 #pragma unroll
     for (int i = SEG - 1; i >= 0; i--) {
         if (b + i < e) {
-            const double c = (b + i == m - 1) ? 0.0 : cpv[b + i];
+            const double c = (b + i == m - 1) ? 0.0 : thomas_factor(S, inv, m, b + i);
             C = v[i] - c * C;
             Dm = -c * Dm;
         }
@@ -649,7 +660,7 @@ __device__ __forceinline__ void solve_strided_lines(const SolveParams<T> &S, uin
 #pragma unroll
     for (int i = SEG - 1; i >= 0; i--) {
         if (b + i < e) {
-            const double c = (b + i == m - 1) ? 0.0 : cpv[b + i];
+            const double c = (b + i == m - 1) ? 0.0 : thomas_factor(S, inv, m, b + i);
             z = v[i] - c * z;
             v[i] = z;
         }
@@ -779,7 +790,7 @@ static int copy_box(const T *src, uint64_t ss0, uint64_t ss1, T *dst, uint64_t d
 
 // ---- grouped kernels (batched refactor) ---------------------------------------
 // Each walks the virtual workgroups of a grouped launch (see GroupLaunch):
-// the entry's parameters are copied to scalar registers and the same tile /
+// the entry's parameters are read at wave-uniform addresses and the same tile /
 // line body as the single-field kernel runs, so every field's results equal
 // its single-field decompose bit for bit.
 template <class T> struct SolveEntry {
@@ -906,6 +917,22 @@ WsLayout ws_layout(const LevelGeom &g, size_t elem) {
     return w;
 }
 
+// factor table of level l, dim d on the device and its fixed-point summary
+template <class T>
+void set_factors(SolveParams<T> &S, const double *tab_dev, const Factors &F, int l, int d, uint32_t m) {
+    const double *h = F.host.data() + F.off[l][d];
+    S.inv = tab_dev + F.off[l][d];
+    S.last = h[m - 1];
+    S.nfix = m;
+    S.lim = 0.0;
+    if (m >= 3) {
+        uint32_t j = m - 2;  // h[j .. m - 2] all equal h[m - 2]
+        while (j > 0 && h[j - 1] == h[m - 2]) j--;
+        S.nfix = j;
+        S.lim = h[m - 2];
+    }
+}
+
 template <class T>
 int solve_all(const LevelGeom &g, int l, double *f, const double *tab_dev, const Factors &F, int epi, const T *uc_in,
               T *uc_out, hipStream_t s) {
@@ -921,8 +948,7 @@ int solve_all(const LevelGeom &g, int l, double *f, const double *tab_dev, const
         S.f = f;
         S.fp = f_pitch(c[2]);
         S.c[0] = c[0]; S.c[1] = c[1]; S.c[2] = c[2];
-        S.inv = tab_dev + F.off[l][d];
-        S.cp = S.inv + c[d];
+        set_factors(S, tab_dev, F, l, d, c[d]);
         S.epi = (d == last) ? epi : kEpiNone;
         S.uc_in = uc_in;
         S.uc_out = uc_out;
@@ -1276,8 +1302,7 @@ int mgard_batched_launches(const std::vector<BatchFieldRef> &fields, const std::
                     E.S.f = fv;
                     E.S.fp = f_pitch(c[2]);
                     E.S.c[0] = c[0]; E.S.c[1] = c[1]; E.S.c[2] = c[2];
-                    E.S.inv = tab + F.off[l][d];
-                    E.S.cp = E.S.inv + c[d];
+                    set_factors(E.S, tab, F, l, d, c[d]);
                     E.S.epi = (d == last) ? kEpiAdd : kEpiNone;
                     E.S.uc_in = uc;
                     E.S.uc_out = uc;
