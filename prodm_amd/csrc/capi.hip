@@ -270,10 +270,24 @@ static int plan_alloc(void **p, size_t bytes, uint64_t &total) {
 static void plan_free(prodm_mdr_plan *p) {
     if (p->g_ref) (void)hipGraphExecDestroy(p->g_ref);
     if (p->g_ret) (void)hipGraphExecDestroy(p->g_ret);
+    for (hipEvent_t e : p->lev_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (p->join_ev) (void)hipEventDestroy(p->join_ev);
+    if (p->side) (void)hipStreamDestroy(p->side);
     void *ptrs[] = {p->ws, p->levels, p->keys, p->planes, p->rb, p->words, p->signs, p->gbws};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
 }
+
+static int plan_side(prodm_mdr_plan *p) {
+    if (p->side) return kOk;
+    PRODM_HIP(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
+    for (int l = 0; l <= p->g.L; l++) PRODM_HIP(hipEventCreateWithFlags(&p->lev_ev[l], hipEventDisableTiming));
+    PRODM_HIP(hipEventCreateWithFlags(&p->join_ev, hipEventDisableTiming));
+    return kOk;
+}
+
+template <class F> static int hook_call(void *ctx, int level) { return (*static_cast<F *>(ctx))(level); }
 
 template <class T> static int plan_refactor(prodm_mdr_plan *p, const T *in, hipStream_t s) {
     using S = typename Traits<T>::Stream;
@@ -288,18 +302,43 @@ template <class T> static int plan_refactor(prodm_mdr_plan *p, const T *in, hipS
     const uint64_t st[3] = {(uint64_t)g.dims[1] * g.dims[2], g.dims[2], 1};
     StageTimer *tm = p->timer.on ? &p->timer : nullptr;
     if (tm) tm->stream = s;
-    rc = mgard_decompose<T>(in, st, g, sink, keys, p->ws, p->ws_bytes, s, tm, true, p->hier != 0);
-    if (rc) return rc;
-    for (int l = g.L; l >= 0; l--) {
+    auto encode_level = [&](int l, hipStream_t es) -> int {
         StageScope sc(tm, l == g.L ? kStEncTop : kStEncRest);
         S *planes = static_cast<S *>(p->planes) + p->plane_off[l];
         if (p->encoder == PRODM_ENC_NEGABINARY)
-            rc = launch_nb_encode<T>(sink.level[l], (uint32_t)g.elems[l], 0, keys + l, p->nb, planes, p->plane_stride[l], s);
-        else
-            rc = launch_gb_encode<T>(sink.level[l], (uint32_t)g.elems[l], 0, keys + l, p->nb, 0, p->rb + p->rb_off[l],
-                                     planes, p->plane_stride[l], p->words + (size_t)l * p->nb, p->gbws, p->gbws_bytes, s);
-        if (rc) return rc;
+            return launch_nb_encode<T>(sink.level[l], (uint32_t)g.elems[l], 0, keys + l, p->nb, planes,
+                                       p->plane_stride[l], es);
+        return launch_gb_encode<T>(sink.level[l], (uint32_t)g.elems[l], 0, keys + l, p->nb, 0, p->rb + p->rb_off[l],
+                                   planes, p->plane_stride[l], p->words + (size_t)l * p->nb, p->gbws, p->gbws_bytes, es);
+    };
+    // per-stage timing keeps every launch on one stream; otherwise each
+    // level's encode runs on the side stream as soon as its coefficients and
+    // max are final, beside the decomposition of the coarser levels
+    if (tm || p->capturing) {
+        rc = mgard_decompose<T>(in, st, g, sink, keys, p->ws, p->ws_bytes, s, tm, true, p->hier != 0);
+        for (int l = g.L; l >= 0 && !rc; l--) rc = encode_level(l, s);
+        return rc;
     }
+    if ((rc = plan_side(p))) return rc;
+    // the finest levels' encodes start once the decomposition reaches level
+    // L - 2, whose short, latency-bound launches leave the GPU room for them
+    // (beside the finest levels' own solves they would only compete for HBM)
+    const int fork = std::max(0, g.L - 2);
+    auto ready = [&](int l) -> int {
+        if (l > fork) return kOk;
+        PRODM_HIP(hipEventRecord(p->lev_ev[l], s));
+        PRODM_HIP(hipStreamWaitEvent(p->side, p->lev_ev[l], 0));
+        for (int j = (l == fork ? g.L : l); j >= l; j--)
+            if (int r = encode_level(j, p->side)) return r;
+        return kOk;
+    };
+    const LevelHook hook{&hook_call<decltype(ready)>, &ready};
+    rc = mgard_decompose<T>(in, st, g, sink, keys, p->ws, p->ws_bytes, s, nullptr, true, p->hier != 0, &hook);
+    // join the side stream back, also after an error (nothing stays forked)
+    const hipError_t e1 = hipEventRecord(p->join_ev, p->side);
+    const hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent(s, p->join_ev, 0) : e1;
+    if (rc) return rc;
+    if (e2 != hipSuccess) { set_error(std::string("refactor join: ") + hipGetErrorString(e2)); return kErrHip; }
     return kOk;
 }
 
@@ -310,27 +349,52 @@ template <class T> static int plan_retrieve(prodm_mdr_plan *p, const uint8_t *k,
     K *keys = static_cast<K *>(p->keys);
     CoefSink<T> src;
     for (int l = 0; l <= g.L; l++) src.level[l] = static_cast<T *>(p->levels) + p->level_off[l];
-    int rc = kOk;
+    for (int l = 0; l <= g.L; l++)
+        if (k[l] > p->nb) { set_error("retrieve: more planes than encoded"); return kErrArg; }
     StageTimer *tm = p->timer.on ? &p->timer : nullptr;
     if (tm) tm->stream = s;
-    for (int l = g.L; l >= 0; l--) {
+    auto decode_level = [&](int l, hipStream_t ds) -> int {
         StageScope sc(tm, l == g.L ? kStDecodeTop : kStDecodeRest);
-        if (k[l] > p->nb) { set_error("retrieve: more planes than encoded"); return kErrArg; }
         S *planes = static_cast<S *>(p->planes) + p->plane_off[l];
         PlanePtrs pp{};
         for (int i = 0; i < k[l]; i++) pp.p[i] = planes + (uint64_t)i * p->plane_stride[l];
         const uint32_t n = (uint32_t)g.elems[l];
-        if (p->encoder == PRODM_ENC_NEGABINARY) {
-            rc = launch_nb_decode<T>(pp, n, 0, keys + l, 0, k[l], src.level[l], s);
-        } else {
-            S *signs = static_cast<S *>(p->signs) + p->rb_off[l];
-            rc = launch_gb_decode<T>(pp, n, 0, keys + l, 0, k[l], p->rb + p->rb_off[l], signs, src.level[l], p->gbws,
-                                     p->gbws_bytes, 0, s);
-        }
-        if (rc) return rc;
-    }
+        if (p->encoder == PRODM_ENC_NEGABINARY) return launch_nb_decode<T>(pp, n, 0, keys + l, 0, k[l], src.level[l], ds);
+        S *signs = static_cast<S *>(p->signs) + p->rb_off[l];
+        return launch_gb_decode<T>(pp, n, 0, keys + l, 0, k[l], p->rb + p->rb_off[l], signs, src.level[l], p->gbws,
+                                   p->gbws_bytes, 0, ds);
+    };
     const uint64_t st[3] = {(uint64_t)g.dims[1] * g.dims[2], g.dims[2], 1};
-    return mgard_recompose<T>(src, g, 1, out, st, p->ws, p->ws_bytes, s, tm, p->hier != 0);
+    int rc = kOk;
+    if (tm || p->capturing) {
+        for (int l = g.L; l >= 0 && !rc; l--) rc = decode_level(l, s);
+        if (rc) return rc;
+        return mgard_recompose<T>(src, g, 1, out, st, p->ws, p->ws_bytes, s, tm, p->hier != 0);
+    }
+    // decodes on the side stream, coarsest first; the recomposition waits
+    // for each level's decode just before it first reads that level
+    if ((rc = plan_side(p))) return rc;
+    PRODM_HIP(hipEventRecord(p->join_ev, s));
+    PRODM_HIP(hipStreamWaitEvent(p->side, p->join_ev, 0));
+    int l = 0;
+    for (; l <= g.L && !rc; l++) {
+        rc = decode_level(l, p->side);
+        if (!rc && hipEventRecord(p->lev_ev[l], p->side) != hipSuccess) { set_error("retrieve: event record"); rc = kErrHip; }
+    }
+    if (rc) {
+        // join whatever was forked before reporting the error
+        if (hipEventRecord(p->join_ev, p->side) == hipSuccess) (void)hipStreamWaitEvent(s, p->join_ev, 0);
+        return rc;
+    }
+    auto need = [&](int lv) -> int {
+        PRODM_HIP(hipStreamWaitEvent(s, p->lev_ev[lv], 0));
+        return kOk;
+    };
+    const LevelHook hook{&hook_call<decltype(need)>, &need};
+    rc = mgard_recompose<T>(src, g, 1, out, st, p->ws, p->ws_bytes, s, nullptr, p->hier != 0, &hook);
+    // every decode is joined (the last level is waited for before its recomposition)
+    if (rc) (void)hipStreamWaitEvent(s, p->lev_ev[g.L], 0);
+    return rc;
 }
 
 }  // namespace prodm
@@ -776,7 +840,9 @@ int prodm_mdr_refactor_graph(prodm_mdr_plan *p, const void *d_in, prodm_stream_t
     if (!p->g_ref || p->g_ref_in != d_in) {
         const bool on = p->timer.on;
         p->timer.on = false;  // no event nodes in the graph
+        p->capturing = true;  // one stream: a forked graph replays slower than the eager fork
         const int rc = prodm::graph_capture(s, &p->g_ref, [&] { return prodm_mdr_refactor(p, d_in, stream); });
+        p->capturing = false;
         p->timer.on = on;
         if (rc) return rc;
         p->g_ref_in = d_in;
@@ -791,7 +857,9 @@ int prodm_mdr_retrieve_graph(prodm_mdr_plan *p, const uint8_t *k, void *d_out, p
     if (!p->g_ret || p->g_ret_out != d_out || std::memcmp(p->g_ret_k, k, p->g.L + 1) != 0) {
         const bool on = p->timer.on;
         p->timer.on = false;
+        p->capturing = true;
         const int rc = prodm::graph_capture(s, &p->g_ret, [&] { return prodm_mdr_retrieve(p, k, d_out, stream); });
+        p->capturing = false;
         p->timer.on = on;
         if (rc) return rc;
         p->g_ret_out = d_out;

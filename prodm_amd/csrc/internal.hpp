@@ -108,13 +108,22 @@ template <class T> struct CoefSink {
 // uploaded once by mgard_upload_tables (synchronous); otherwise each call
 // uploads them on the stream.
 int mgard_upload_tables(const LevelGeom &g, size_t elem, void *ws, size_t ws_bytes);
+// Level hook: decompose calls it with l once the launches that finish level
+// l's coefficients and max key are on the stream; recompose calls it with l
+// before the first launch that reads level l (for cross-stream events).
+struct LevelHook {
+    int (*fn)(void *ctx, int level);
+    void *ctx;
+    int operator()(int l) const { return fn ? fn(ctx, l) : kOk; }
+};
 template <class T>
 int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g, const CoefSink<T> &sink,
                     KeyOf<T> *level_keys, void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm = nullptr,
-                    bool tables_ready = false, bool hier = false);
+                    bool tables_ready = false, bool hier = false, const LevelHook *hook = nullptr);
 template <class T>
 int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int tables_ready, T *out, const uint64_t *out_strides,
-                    void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm = nullptr, bool hier = false);
+                    void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm = nullptr, bool hier = false,
+                    const LevelHook *hook = nullptr);
 // Transpose of recompose (adjoint.hip): g_in = dJ/du on the level-L box
 // (strides in_strides), out = dJ/dcoefficients in the decompose layout.
 size_t adjoint_workspace_bytes(const LevelGeom &g);

@@ -1028,7 +1028,9 @@ int mgard_upload_tables(const LevelGeom &g, size_t elem, void *ws, size_t ws_byt
 template <class T>
 int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g, const CoefSink<T> &sink,
                     KeyOf<T> *level_keys, void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm, bool tables_ready,
-                    bool hier) {
+                    bool hier, const LevelHook *hook) {
+    const LevelHook none{nullptr, nullptr};
+    const LevelHook &ready = hook ? *hook : none;
     const WsLayout w = ws_layout(g, sizeof(T));
     if (ws_bytes < w.total) { set_error("decompose: workspace too small"); return kErrArg; }
     char *base = static_cast<char *>(ws);
@@ -1060,6 +1062,7 @@ int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g,
             StageScope sc(tm, l == g.L ? kStDecLoadTop : kStDecLoadRest);
             rc = launch_load<T, kDecompose>(P, ilv, s);
         }
+        if (!rc) rc = ready(l);
         if (rc) return rc;
         if (!hier) {
             StageScope sc(tm, l == g.L ? kStDecSolveTop : kStDecSolveRest);
@@ -1075,13 +1078,13 @@ int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g,
     StageScope sc0(tm, kStDecCopy);
     const uint32_t *c0 = g.ld[0];
     if (ilv) {
-        if (g.L == 0) return copy_box<T>(in, in_strides[0], in_strides[1], sink.level[0], (uint64_t)c0[1] * c0[2], c0[2], c0, s);
+        int rc;
         // a kernel rather than a copy node, so that the sequence captures
         // into a graph as kernel nodes only
-        int rc = copy_box<T>(cur, cs[0], cs[1], sink.level[0], (uint64_t)c0[1] * c0[2], c0[2], c0, s);
-        if (rc) return rc;
-        if (level_keys) return launch_maxabs<T>(sink.level[0], points3(c0), level_keys + 0, s);
-        return kOk;
+        if (g.L == 0) rc = copy_box<T>(in, in_strides[0], in_strides[1], sink.level[0], (uint64_t)c0[1] * c0[2], c0[2], c0, s);
+        else rc = copy_box<T>(cur, cs[0], cs[1], sink.level[0], (uint64_t)c0[1] * c0[2], c0[2], c0, s);
+        if (!rc && level_keys) rc = launch_maxabs<T>(sink.level[0], points3(c0), level_keys + 0, s);
+        return rc ? rc : ready(0);
     }
     int rc = copy_box<T>(cur, cs[0], cs[1], sink.full, sink.s[0], sink.s[1], c0, s);
     if (rc) return rc;
@@ -1092,13 +1095,17 @@ int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g,
     return kOk;
 }
 template int mgard_decompose<float>(const float *, const uint64_t *, const LevelGeom &, const CoefSink<float> &,
-                                    uint32_t *, void *, size_t, hipStream_t, StageTimer *, bool, bool);
+                                    uint32_t *, void *, size_t, hipStream_t, StageTimer *, bool, bool,
+                                    const LevelHook *);
 template int mgard_decompose<double>(const double *, const uint64_t *, const LevelGeom &, const CoefSink<double> &,
-                                     unsigned long long *, void *, size_t, hipStream_t, StageTimer *, bool, bool);
+                                     unsigned long long *, void *, size_t, hipStream_t, StageTimer *, bool, bool,
+                                     const LevelHook *);
 
 template <class T>
 int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int tables_ready, T *out, const uint64_t *out_strides,
-                    void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm, bool hier) {
+                    void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm, bool hier, const LevelHook *hook) {
+    const LevelHook none{nullptr, nullptr};
+    const LevelHook &need = hook ? *hook : none;
     const WsLayout w = ws_layout(g, sizeof(T));
     if (ws_bytes < w.total) { set_error("recompose: workspace too small"); return kErrArg; }
     char *base = static_cast<char *>(ws);
@@ -1110,6 +1117,7 @@ int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int tables_ready
         PRODM_HIP(hipMemcpyAsync(tab, F.host.data(), F.host.size() * 8, hipMemcpyHostToDevice, s));
     const bool ilv = src.full == nullptr;
     const uint32_t *c0 = g.ld[0];
+    if (int rc = need(0)) return rc;
     if (g.L == 0) {
         if (ilv) return copy_box<T>(src.level[0], (uint64_t)c0[1] * c0[2], c0[2], out, out_strides[0], out_strides[1], c0, s);
         return copy_box<T>(src.full, src.s[0], src.s[1], out, out_strides[0], out_strides[1], c0, s);
@@ -1122,6 +1130,7 @@ int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int tables_ready
     else rc = copy_box<T>(src.full, src.s[0], src.s[1], cur, (uint64_t)c0[1] * c0[2], c0[2], c0, s);
     if (rc) return rc;
     for (int l = 1; l <= g.L; l++) {
+        if ((rc = need(l))) return rc;
         LoadParams<T> P = load_params(g, l, src);
         P.f = f;
         P.fp = f_pitch(P.c[2]);
@@ -1186,9 +1195,9 @@ const double *mgard_adjoint_solution(const LevelGeom &g, int l, void *ws, uint32
 }
 
 template int mgard_recompose<float>(const CoefSink<float> &, const LevelGeom &, int, float *, const uint64_t *, void *,
-                                    size_t, hipStream_t, StageTimer *, bool);
+                                    size_t, hipStream_t, StageTimer *, bool, const LevelHook *);
 template int mgard_recompose<double>(const CoefSink<double> &, const LevelGeom &, int, double *, const uint64_t *,
-                                     void *, size_t, hipStream_t, StageTimer *, bool);
+                                     void *, size_t, hipStream_t, StageTimer *, bool, const LevelHook *);
 
 // ---- batched decompose: grouped launches over many fields ---------------------
 namespace {

@@ -33,6 +33,13 @@ struct prodm_mdr_plan {
     const void *g_ref_in = nullptr;
     void *g_ret_out = nullptr;
     uint8_t g_ret_k[33] = {0};
+    // second stream: the encode of a level overlaps the decomposition of the
+    // coarser ones, and the decode of a level the recomposition of the finer
+    // ones (events per level, created on first use)
+    hipStream_t side = nullptr;
+    hipEvent_t lev_ev[33] = {};
+    hipEvent_t join_ev = nullptr;
+    bool capturing = false;  // graph capture records the single-stream order
 };
 
 namespace prodm {
