@@ -510,76 +510,121 @@ template <class T> __device__ inline void epilogue(const SolveParams<T> &S, uint
     else ((PRODM_GLOBAL double *)S.f)[fidx] = z;
 }
 
-// contiguous lines (dim 2): one wave per line, lane holds a segment of SEG;
-// wave w of the workgroup solves line line0 + w
+// contiguous lines (dim 2): a wave solves kContigLPW lines side by side (two
+// independent chains interleave, hiding each other's latency), each lane
+// holding a segment of at most SEG of every line; wave w of the workgroup
+// solves lines line0 + kContigLPW w + p
+constexpr int kContigLPW = 2;
 template <class T, int SEG>
 __device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint64_t nlines, uint64_t line0) {
+    constexpr int NL = kContigLPW;
     const uint32_t m = S.c[2];
     const uint32_t lane = lane_id(), wv = threadIdx.x / 64;
-    const uint64_t line = line0 + wv;
-    if (line >= nlines) return;
-    const PRODM_GLOBAL double *x = (const PRODM_GLOBAL double *)S.f + line * S.fp;
+    const uint64_t first = line0 + (uint64_t)wv * NL;
+    if (first >= nlines) return;
     const PRODM_GLOBAL double *inv = (const PRODM_GLOBAL double *)S.inv;
     const uint32_t seg = (m + 63) / 64;
     const uint32_t b = lane * seg;
     const uint32_t e = min(m, b + seg);
-    double v[SEG];
+    bool ok[NL];
+    double v[NL][SEG];
 #pragma unroll
-    for (int i = 0; i < SEG; i++) v[i] = (b + i < e) ? x[b + i] : 0.0;
+    for (int p = 0; p < NL; p++) {
+        ok[p] = first + p < nlines;
+        const PRODM_GLOBAL double *x = (const PRODM_GLOBAL double *)S.f + (first + p) * S.fp;
+#pragma unroll
+        for (int i = 0; i < SEG; i++) v[p][i] = (ok[p] && b + i < e) ? x[b + i] : 0.0;
+    }
     // forward: y_i = (f_i - y_{i-1}) * inv_i  == affine map y_{i-1} -> y_i
-    double A = 0.0, Bm = 1.0;
+    double A[NL], Bm[NL];
+#pragma unroll
+    for (int p = 0; p < NL; p++) { A[p] = 0.0; Bm[p] = 1.0; }
 #pragma unroll
     for (int i = 0; i < SEG; i++) {
         if (b + i < e) {
             const double iv = thomas_factor(S, inv, m, b + i);
-            A = (v[i] - A) * iv;
-            Bm = -Bm * iv;
+#pragma unroll
+            for (int p = 0; p < NL; p++) {
+                A[p] = (v[p][i] - A[p]) * iv;
+                Bm[p] = -Bm[p] * iv;
+            }
         }
     }
     for (int o = 1; o < 64; o <<= 1) {
-        const double pA = __shfl_up(A, o, 64), pB = __shfl_up(Bm, o, 64);
-        if ((int)lane >= o) { A = A + Bm * pA; Bm = Bm * pB; }
+#pragma unroll
+        for (int p = 0; p < NL; p++) {
+            const double pA = __shfl_up(A[p], o, 64), pB = __shfl_up(Bm[p], o, 64);
+            if ((int)lane >= o) { A[p] = A[p] + Bm[p] * pA; Bm[p] = Bm[p] * pB; }
+        }
     }
-    double y = __shfl_up(A, 1, 64);
-    if (lane == 0) y = 0.0;
+    double y[NL];
+#pragma unroll
+    for (int p = 0; p < NL; p++) {
+        y[p] = __shfl_up(A[p], 1, 64);
+        if (lane == 0) y[p] = 0.0;
+    }
 #pragma unroll
     for (int i = 0; i < SEG; i++) {
         if (b + i < e) {
-            y = (v[i] - y) * thomas_factor(S, inv, m, b + i);
-            v[i] = y;
+            const double iv = thomas_factor(S, inv, m, b + i);
+#pragma unroll
+            for (int p = 0; p < NL; p++) {
+                y[p] = (v[p][i] - y[p]) * iv;
+                v[p][i] = y[p];
+            }
         }
     }
     // backward: z_i = y_i - cp_i z_{i+1} (z_{m-1} = y_{m-1})
-    double C = 0.0, Dm = 1.0;
+    double Cc[NL], Dm[NL];
+#pragma unroll
+    for (int p = 0; p < NL; p++) { Cc[p] = 0.0; Dm[p] = 1.0; }
 #pragma unroll
     for (int i = SEG - 1; i >= 0; i--) {
         if (b + i < e) {
             const double c = (b + i == m - 1) ? 0.0 : thomas_factor(S, inv, m, b + i);
-            C = v[i] - c * C;
-            Dm = -c * Dm;
+#pragma unroll
+            for (int p = 0; p < NL; p++) {
+                Cc[p] = v[p][i] - c * Cc[p];
+                Dm[p] = -c * Dm[p];
+            }
         }
     }
     for (int o = 1; o < 64; o <<= 1) {
-        const double nC = __shfl_down(C, o, 64), nD = __shfl_down(Dm, o, 64);
-        if (lane + o < 64) { C = C + Dm * nC; Dm = Dm * nD; }
+#pragma unroll
+        for (int p = 0; p < NL; p++) {
+            const double nC = __shfl_down(Cc[p], o, 64), nD = __shfl_down(Dm[p], o, 64);
+            if (lane + o < 64) { Cc[p] = Cc[p] + Dm[p] * nC; Dm[p] = Dm[p] * nD; }
+        }
     }
-    double z = __shfl_down(C, 1, 64);
-    if (lane == 63) z = 0.0;
+    double z[NL];
+#pragma unroll
+    for (int p = 0; p < NL; p++) {
+        z[p] = __shfl_down(Cc[p], 1, 64);
+        if (lane == 63) z[p] = 0.0;
+    }
 #pragma unroll
     for (int i = SEG - 1; i >= 0; i--) {
         if (b + i < e) {
             const double c = (b + i == m - 1) ? 0.0 : thomas_factor(S, inv, m, b + i);
-            z = v[i] - c * z;
-            v[i] = z;
+#pragma unroll
+            for (int p = 0; p < NL; p++) {
+                z[p] = v[p][i] - c * z[p];
+                v[p][i] = z[p];
+            }
         }
     }
 #pragma unroll
-    for (int i = 0; i < SEG; i++)
-        if (b + i < e) epilogue(S, line * S.fp + b + i, line * m + b + i, v[i]);
+    for (int p = 0; p < NL; p++) {
+        if (!ok[p]) continue;
+        const uint64_t line = first + p;
+#pragma unroll
+        for (int i = 0; i < SEG; i++)
+            if (b + i < e) epilogue(S, line * S.fp + b + i, line * m + b + i, v[p][i]);
+    }
 }
 template <class T, int SEG>
 __global__ __launch_bounds__(256) void k_solve_contig(SolveParams<T> S, uint64_t nlines) {
-    solve_contig_lines<T, SEG>(S, nlines, (uint64_t)blockIdx.x * 4);
+    solve_contig_lines<T, SEG>(S, nlines, (uint64_t)blockIdx.x * 4 * kContigLPW);
 }
 
 // strided lines along dim d (1 or 0) of the coarse grid, lane per line.
@@ -829,7 +874,7 @@ __global__ __launch_bounds__(512) void k_contig_grouped(const SolveEntry<T> *__r
         const uint32_t e = group_entry(first, nent, vb);
         const SolveEntry<T> E = uniform_load(ps + e);
         const uint32_t f0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)first[e]);
-        solve_contig_lines<T, SEG>(E.S, E.nlines, (uint64_t)(vb - f0) * wpb);
+        solve_contig_lines<T, SEG>(E.S, E.nlines, (uint64_t)(vb - f0) * wpb * kContigLPW);
     }
 }
 
@@ -955,7 +1000,7 @@ int solve_all(const LevelGeom &g, int l, double *f, const double *tab_dev, const
         const uint32_t m = c[d];
         if (d == 2) {
             const uint64_t nlines = (uint64_t)c[0] * c[1];
-            const uint32_t grid = div_up(nlines, 4);
+            const uint32_t grid = div_up(nlines, 4 * kContigLPW);
             const uint32_t seg = (m + 63) / 64;
             if (seg <= 4) hipLaunchKernelGGL((k_solve_contig<T, 4>), dim3(grid), dim3(256), 0, s, S, nlines);
             else if (seg <= 8) hipLaunchKernelGGL((k_solve_contig<T, 8>), dim3(grid), dim3(256), 0, s, S, nlines);
@@ -1339,7 +1384,7 @@ int mgard_batched_launches(const std::vector<BatchFieldRef> &fields, const std::
                 uint32_t mmax = 0;
                 for (auto &x : se[d]) {
                     mmax = std::max(mmax, x.second.m);
-                    x.first = div_up(x.second.nlines, wpb);
+                    x.first = div_up(x.second.nlines, (uint64_t)wpb * kContigLPW);
                 }
                 L.variant = seg_class_contig(mmax);
                 if (!L.variant) { set_error("line too long for the contiguous solve"); return kErrUnsupported; }
