@@ -274,7 +274,7 @@ static void plan_free(prodm_mdr_plan *p) {
         if (e) (void)hipEventDestroy(e);
     if (p->join_ev) (void)hipEventDestroy(p->join_ev);
     if (p->side) (void)hipStreamDestroy(p->side);
-    void *ptrs[] = {p->ws, p->levels, p->keys, p->planes, p->rb, p->words, p->signs, p->gbws};
+    void *ptrs[] = {p->ws, p->levels, p->keys, p->planes, p->rb, p->words, p->signs, p->gbws, p->flev};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
 }
@@ -371,14 +371,29 @@ template <class T> static int plan_retrieve(prodm_mdr_plan *p, const uint8_t *k,
         if (rc) return rc;
         return mgard_recompose<T>(src, g, 1, out, st, p->ws, p->ws_bytes, s, tm, p->hier != 0);
     }
-    // decodes on the side stream, coarsest first; the recomposition waits
-    // for each level's decode just before it first reads that level
+    // on the side stream, coarsest first: each level's decode and, for the
+    // orthogonal decomposer, its recompose load sweep into a per-level load
+    // vector (it reads only that level's coefficients); the main stream's
+    // chain of solves and interpolations waits for level l just before it
     if ((rc = plan_side(p))) return rc;
+    const bool ahead = p->hier == 0 && g.L >= 1;
+    double *fl[33] = {nullptr};
+    if (ahead) {
+        if (!p->flev) {
+            uint64_t total = 0;
+            for (int l = 1; l <= g.L; l++) {
+                p->flev_off[l] = total;
+                total += (mgard_level_f_bytes(g, l) + 255) & ~(uint64_t)255;
+            }
+            if ((rc = plan_alloc(&p->flev, total, p->device_bytes))) return rc;
+        }
+        for (int l = 1; l <= g.L; l++) fl[l] = reinterpret_cast<double *>(static_cast<char *>(p->flev) + p->flev_off[l]);
+    }
     PRODM_HIP(hipEventRecord(p->join_ev, s));
     PRODM_HIP(hipStreamWaitEvent(p->side, p->join_ev, 0));
-    int l = 0;
-    for (; l <= g.L && !rc; l++) {
+    for (int l = 0; l <= g.L && !rc; l++) {
         rc = decode_level(l, p->side);
+        if (!rc && ahead && l >= 1) rc = mgard_recompose_load<T>(src, g, l, fl[l], p->side);
         if (!rc && hipEventRecord(p->lev_ev[l], p->side) != hipSuccess) { set_error("retrieve: event record"); rc = kErrHip; }
     }
     if (rc) {
@@ -391,7 +406,8 @@ template <class T> static int plan_retrieve(prodm_mdr_plan *p, const uint8_t *k,
         return kOk;
     };
     const LevelHook hook{&hook_call<decltype(need)>, &need};
-    rc = mgard_recompose<T>(src, g, 1, out, st, p->ws, p->ws_bytes, s, nullptr, p->hier != 0, &hook);
+    rc = mgard_recompose<T>(src, g, 1, out, st, p->ws, p->ws_bytes, s, nullptr, p->hier != 0, &hook,
+                            ahead ? fl : nullptr);
     // every decode is joined (the last level is waited for before its recomposition)
     if (rc) (void)hipStreamWaitEvent(s, p->lev_ev[g.L], 0);
     return rc;

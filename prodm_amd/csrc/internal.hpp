@@ -120,10 +120,17 @@ template <class T>
 int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g, const CoefSink<T> &sink,
                     KeyOf<T> *level_keys, void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm = nullptr,
                     bool tables_ready = false, bool hier = false, const LevelHook *hook = nullptr);
+// f_levels (optional): per level l the load vector of that level, already
+// computed by mgard_recompose_load (so those sweeps can run ahead on another
+// stream); the recompose then only solves and interpolates.
 template <class T>
 int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int tables_ready, T *out, const uint64_t *out_strides,
                     void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm = nullptr, bool hier = false,
-                    const LevelHook *hook = nullptr);
+                    const LevelHook *hook = nullptr, double *const *f_levels = nullptr);
+// the recompose load sweep of level l (reads only level l's coefficients)
+// into f_l (rows of pitch round16(c2)); bytes of such a buffer
+template <class T> int mgard_recompose_load(const CoefSink<T> &src, const LevelGeom &g, int l, double *f_l, hipStream_t s);
+size_t mgard_level_f_bytes(const LevelGeom &g, int l);
 // Transpose of recompose (adjoint.hip): g_in = dJ/du on the level-L box
 // (strides in_strides), out = dJ/dcoefficients in the decompose layout.
 size_t adjoint_workspace_bytes(const LevelGeom &g);

@@ -1148,7 +1148,8 @@ template int mgard_decompose<double>(const double *, const uint64_t *, const Lev
 
 template <class T>
 int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int tables_ready, T *out, const uint64_t *out_strides,
-                    void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm, bool hier, const LevelHook *hook) {
+                    void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm, bool hier, const LevelHook *hook,
+                    double *const *f_levels) {
     const LevelHook none{nullptr, nullptr};
     const LevelHook &need = hook ? *hook : none;
     const WsLayout w = ws_layout(g, sizeof(T));
@@ -1177,16 +1178,17 @@ int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int tables_ready
     for (int l = 1; l <= g.L; l++) {
         if ((rc = need(l))) return rc;
         LoadParams<T> P = load_params(g, l, src);
-        P.f = f;
+        double *fl = f_levels ? f_levels[l] : f;
+        P.f = fl;
         P.fp = f_pitch(P.c[2]);
         if (!hier) {
-            {
+            if (!f_levels) {
                 StageScope sc(tm, l == g.L ? kStRecLoadTop : kStRecLoadRest);
                 rc = launch_load<T, kRecompose>(P, ilv, s);
             }
             if (rc) return rc;
             StageScope sc(tm, l == g.L ? kStRecSolveTop : kStRecSolveRest);
-            rc = solve_all<T>(g, l, f, tab, F, kEpiSub, cur, cur, s);  // u_c = u_c' - z, in place
+            rc = solve_all<T>(g, l, fl, tab, F, kEpiSub, cur, cur, s);  // u_c = u_c' - z, in place
             if (rc) return rc;
         }
         const uint32_t *n = g.ld[l];
@@ -1240,9 +1242,26 @@ const double *mgard_adjoint_solution(const LevelGeom &g, int l, void *ws, uint32
 }
 
 template int mgard_recompose<float>(const CoefSink<float> &, const LevelGeom &, int, float *, const uint64_t *, void *,
-                                    size_t, hipStream_t, StageTimer *, bool, const LevelHook *);
+                                    size_t, hipStream_t, StageTimer *, bool, const LevelHook *, double *const *);
 template int mgard_recompose<double>(const CoefSink<double> &, const LevelGeom &, int, double *, const uint64_t *,
-                                     void *, size_t, hipStream_t, StageTimer *, bool, const LevelHook *);
+                                     void *, size_t, hipStream_t, StageTimer *, bool, const LevelHook *,
+                                     double *const *);
+
+size_t mgard_level_f_bytes(const LevelGeom &g, int l) {
+    if (l < 1 || l > g.L) return 0;
+    const uint32_t *c = g.ld[l - 1];
+    return (size_t)c[0] * c[1] * f_pitch(c[2]) * sizeof(double);
+}
+
+template <class T> int mgard_recompose_load(const CoefSink<T> &src, const LevelGeom &g, int l, double *f_l, hipStream_t s) {
+    if (l < 1 || l > g.L || !f_l) { set_error("recompose load: bad level"); return kErrArg; }
+    LoadParams<T> P = load_params(g, l, src);
+    P.f = f_l;
+    P.fp = f_pitch(P.c[2]);
+    return launch_load<T, kRecompose>(P, src.full == nullptr, s);
+}
+template int mgard_recompose_load<float>(const CoefSink<float> &, const LevelGeom &, int, double *, hipStream_t);
+template int mgard_recompose_load<double>(const CoefSink<double> &, const LevelGeom &, int, double *, hipStream_t);
 
 // ---- batched decompose: grouped launches over many fields ---------------------
 namespace {

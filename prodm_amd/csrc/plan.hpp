@@ -40,6 +40,9 @@ struct prodm_mdr_plan {
     hipEvent_t lev_ev[33] = {};
     hipEvent_t join_ev = nullptr;
     bool capturing = false;  // graph capture records the single-stream order
+    // per-level load vectors of the retrieve pipeline (allocated on first use)
+    void *flev = nullptr;
+    uint64_t flev_off[33] = {0};
 };
 
 namespace prodm {
