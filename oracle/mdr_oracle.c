@@ -194,6 +194,21 @@ struct orc_gb_state {
     uint8_t **signs;   /* one byte per element */
 };
 
+/* BitDecoder::decode (PerBitBPEncoder.hpp:41-60): next bit, LSB first */
+static inline int pb_next_bit(const uint64_t *stream, uint64_t *pos) {
+    const int b = (int)((stream[*pos / 64] >> (*pos % 64)) & 1u);
+    (*pos)++;
+    return b;
+}
+
+/* PerBit progressive state (PerBitBPEncoder.hpp:400-401: signs and "sign
+ * recorded" flags per element), keyed by level                              */
+struct orc_pb_state {
+    int max_levels;
+    uint32_t *n;
+    uint8_t **signs, **flags;
+};
+
 /* ------------------------------------------------------------------------ */
 /* type-generic part                                                          */
 /* ------------------------------------------------------------------------ */
@@ -358,4 +373,31 @@ void orc_gb_level_errors_f32(const float *x, uint64_t n, int32_t exp, int nb, do
 }
 void orc_gb_level_errors_seq_f32(const float *x, uint64_t n, int32_t exp, int nb, double *out) {
     fused_level_errors(gb_fused_terms, x, n, exp, nb, 0, out);
+}
+
+orc_pb_state *orc_pb_state_new(int max_levels) {
+    orc_pb_state *s = (orc_pb_state *)calloc(1, sizeof(*s));
+    s->max_levels = max_levels;
+    s->n = (uint32_t *)calloc((size_t)max_levels, sizeof(uint32_t));
+    s->signs = (uint8_t **)calloc((size_t)max_levels, sizeof(uint8_t *));
+    s->flags = (uint8_t **)calloc((size_t)max_levels, sizeof(uint8_t *));
+    return s;
+}
+void orc_pb_state_free(orc_pb_state *s) {
+    if (!s) return;
+    for (int i = 0; i < s->max_levels; i++) { free(s->signs[i]); free(s->flags[i]); }
+    free(s->signs); free(s->flags); free(s->n); free(s);
+}
+int orc_pb_state_level(orc_pb_state *s, int level, uint32_t n, uint8_t **signs, uint8_t **flags) {
+    if (level < 0 || level >= s->max_levels) return -1;
+    if (!s->signs[level]) {
+        s->signs[level] = (uint8_t *)calloc(n ? n : 1, 1);
+        s->flags[level] = (uint8_t *)calloc(n ? n : 1, 1);
+        s->n[level] = n;
+    } else if (s->n[level] != n) {
+        return -3;
+    }
+    *signs = s->signs[level];
+    *flags = s->flags[level];
+    return 0;
 }

@@ -87,6 +87,19 @@ int orc_gb_decode_f32(orc_gb_state *s, int level, const uint8_t *const *streams,
 int orc_gb_decode_f64(orc_gb_state *s, int level, const uint8_t *const *streams,
                       uint32_t n, int32_t exp, int start, int k, double *out);
 
+/* ---- PerBit (PerBitBPEncoder.hpp:72-146, 282-391) ------------------------
+ * streams[p]: capacity (2n + 63) / 64 words each; sizes in bytes. Decode
+ * state keyed by level; planes [start, start + k) as deltas.              */
+typedef struct orc_pb_state orc_pb_state;
+orc_pb_state *orc_pb_state_new(int max_levels);
+void orc_pb_state_free(orc_pb_state *s);
+int orc_pb_encode_f32(const float *x, uint32_t n, int32_t exp, int nb, uint64_t *const *streams, uint32_t *sizes);
+int orc_pb_encode_f64(const double *x, uint32_t n, int32_t exp, int nb, uint64_t *const *streams, uint32_t *sizes);
+int orc_pb_decode_f32(orc_pb_state *s, int level, const uint64_t *const *streams, uint32_t n, int32_t exp, int start,
+                      int k, float *out);
+int orc_pb_decode_f64(orc_pb_state *s, int level, const uint64_t *const *streams, uint32_t n, int32_t exp, int start,
+                      int k, double *out);
+
 /* ---- error collectors ---------------------------------------------------- */
 void orc_collect_max_f32(float bound, int nb, double *out);
 void orc_collect_max_f64(double bound, int nb, double *out);

@@ -19,6 +19,7 @@
 #include "MDR/Interleaver/DirectInterleaver.hpp"
 #include "MDR/BitplaneEncoder/GroupedBPEncoder.hpp"
 #include "MDR/BitplaneEncoder/NegaBinaryBPEncoder.hpp"
+#include "MDR/BitplaneEncoder/PerBitBPEncoder.hpp"
 #include "MDR/ErrorCollector/MaxErrorCollector.hpp"
 #include "MDR/ErrorCollector/SquaredErrorCollector.hpp"
 #include "MDR/ErrorEstimator/MaxErrorEstimator.hpp"
@@ -92,6 +93,22 @@ int gb_decode(void *h, int level, const uint8_t *const *streams, int32_t n, int3
     return 0;
 }
 
+template <class TD, class TS>
+int pb_encode(const TD *x, int32_t n, int32_t exp, int nb, uint8_t *out, uint64_t cap, uint32_t *sizes) {
+    MDR::PerBitBPEncoder<TD, TS> enc;
+    std::vector<uint32_t> sz;
+    auto streams = enc.encode(x, n, exp, (uint8_t)nb, sz);
+    uint64_t off = 0;
+    int rc = 0;
+    for (int p = 0; p < nb; p++) {
+        sizes[p] = sz[p];
+        if (off + sz[p] > cap) rc = -1;
+        else memcpy(out + off, streams[p], sz[p]);
+        off += sz[p];
+        free(streams[p]);
+    }
+    return rc;
+}
 std::vector<std::vector<uint32_t>> unflat_u32(const uint32_t *p, int L, const int *counts) {
     std::vector<std::vector<uint32_t>> v;
     for (int i = 0; i < L; i++) { v.emplace_back(p, p + counts[i]); p += counts[i]; }
@@ -180,6 +197,28 @@ int ref_gb_decode_f32(void *h, int level, const uint8_t *const *streams, int32_t
 int ref_gb_decode_f64(void *h, int level, const uint8_t *const *streams, int32_t n, int32_t exp, int start,
                       int k, double *out) {
     return gb_decode<MDR::GroupedBPEncoder<double, uint64_t>>(h, level, streams, n, exp, start, k, out);
+}
+
+// PerBitBPEncoder (PerBitBPEncoder.hpp:72-146, 282-391): streams packed back
+// to back into out (cap bytes), sizes[p] bytes each; decoder handles keep the
+// per-level sign state
+int ref_pb_encode_f32(const float *x, int32_t n, int32_t exp, int nb, uint8_t *out, uint64_t cap, uint32_t *sizes) {
+    return pb_encode<float, uint32_t>(x, n, exp, nb, out, cap, sizes);
+}
+int ref_pb_encode_f64(const double *x, int32_t n, int32_t exp, int nb, uint8_t *out, uint64_t cap, uint32_t *sizes) {
+    return pb_encode<double, uint64_t>(x, n, exp, nb, out, cap, sizes);
+}
+void *ref_pb_new_f32() { return new MDR::PerBitBPEncoder<float, uint32_t>(); }
+void *ref_pb_new_f64() { return new MDR::PerBitBPEncoder<double, uint64_t>(); }
+void ref_pb_free_f32(void *h) { delete static_cast<MDR::PerBitBPEncoder<float, uint32_t> *>(h); }
+void ref_pb_free_f64(void *h) { delete static_cast<MDR::PerBitBPEncoder<double, uint64_t> *>(h); }
+int ref_pb_decode_f32(void *h, int level, const uint8_t *const *streams, int32_t n, int32_t exp, int start, int k,
+                      float *out) {
+    return gb_decode<MDR::PerBitBPEncoder<float, uint32_t>, float>(h, level, streams, n, exp, start, k, out);
+}
+int ref_pb_decode_f64(void *h, int level, const uint8_t *const *streams, int32_t n, int32_t exp, int start, int k,
+                      double *out) {
+    return gb_decode<MDR::PerBitBPEncoder<double, uint64_t>, double>(h, level, streams, n, exp, start, k, out);
 }
 
 // the encoders' level-error overloads (NegaBinaryBPEncoder.hpp:65-125,

@@ -139,6 +139,26 @@ def gen_grouped(out):
         out[f"{name}_full"] = one.decode(0, streams, n, e, 0, nb)
 
 
+def gen_perbit(out):
+    """PerBitBPEncoder streams and progressive decodes (PerBitBPEncoder.hpp:72-391)"""
+    cases = (("f32", np.float32, 32, 1000), ("f32", np.float32, 16, 777), ("f64", np.float64, 64, 1000),
+             ("f64", np.float64, 40, 513), ("f32", np.float32, 32, 1))
+    for i, (t, dt, nb, n) in enumerate(cases):
+        x = codec_input(n, dt, 500 + i)
+        e = level_exp(x)
+        streams = ob.pb_encode(x, e, nb, lib="ref")
+        name = f"pb{i}"
+        out[f"{name}_x"] = x
+        out[f"{name}_exp"] = np.array(e)
+        out[f"{name}_nb"] = np.array(nb)
+        out[f"{name}_sizes"] = np.array([len(s) for s in streams], np.uint32)
+        out[f"{name}_bytes"] = np.frombuffer(b"".join(streams), np.uint8)
+        dec = ob.PBDecoder(dt, lib="ref")
+        for j, (s, k) in enumerate(splits(nb)[1:]):
+            out[f"{name}_dec{j}"] = dec.decode(0, streams[s:s + k], n, e, s, k)
+    out["pb_count"] = np.array(len(cases))
+
+
 def gen_collectors(out):
     bounds = [1.0, 3.7, 0.3, 1e-30]
     for t, dt, nb in (("f32", np.float32, 32), ("f64", np.float64, 64)):
@@ -213,6 +233,7 @@ def main():
     gen_interleave(out)
     gen_negabinary(out)
     gen_grouped(out)
+    gen_perbit(out)
     gen_collectors(out)
     gen_interpreters(out)
     path = os.path.join(HERE, "ref_vectors.npz")

@@ -222,6 +222,74 @@ class GBDecoder:
             pass
 
 
+def pb_encode(x, exp, nb, lib="oracle"):
+    """PerBit streams (PerBitBPEncoder.hpp:72-146): list of nb byte strings."""
+    t = _tp(x.dtype)
+    n = x.size
+    words = (2 * n + 63) // 64 + 1
+    sizes = np.zeros(nb, np.uint32)
+    if lib == "oracle":
+        bufs = [np.zeros(words, np.uint64) for _ in range(nb)]
+        ptrs = (C.c_void_p * nb)(*[b.ctypes.data for b in bufs])
+        rc = getattr(oracle(), "orc_pb_encode_" + t)(x.ctypes.data_as(C.c_void_p), C.c_uint32(n), C.c_int32(exp), nb,
+                                                     ptrs, sizes.ctypes.data_as(C.c_void_p))
+        assert rc == 0, rc
+        return [bufs[p].view(np.uint8)[:int(sizes[p])].tobytes() for p in range(nb)]
+    cap = nb * (words * 8 + 64)
+    out = np.zeros(cap, np.uint8)
+    rc = getattr(ref(), "ref_pb_encode_" + t)(x.ctypes.data_as(C.c_void_p), C.c_int32(n), C.c_int32(exp), nb,
+                                              out.ctypes.data_as(C.c_void_p), C.c_uint64(cap),
+                                              sizes.ctypes.data_as(C.c_void_p))
+    assert rc == 0, rc
+    res, off = [], 0
+    for sz in sizes:
+        res.append(out[off:off + int(sz)].tobytes())
+        off += int(sz)
+    return res
+
+
+class PBDecoder:
+    """Progressive PerBit decoder with per-level sign state (oracle or ref)."""
+
+    def __init__(self, dtype, lib="oracle", max_levels=32):
+        self.t = _tp(dtype)
+        self.dtype = dtype
+        self.lib = lib
+        if lib == "oracle":
+            fn = oracle().orc_pb_state_new
+            fn.restype = C.c_void_p
+            self.h = fn(max_levels)
+        else:
+            fn = getattr(ref(), "ref_pb_new_" + self.t)
+            fn.restype = C.c_void_p
+            self.h = fn()
+
+    def decode(self, level, streams, n, exp, start, k):
+        out = np.zeros(n, self.dtype)
+        # one spare word: the reference's bit reader may load the word after the last bit
+        bufs = [np.frombuffer(s + b"\0" * 8, np.uint8).copy() for s in streams[:k]]
+        ptrs = (C.c_void_p * max(k, 1))(*[b.ctypes.data for b in bufs])
+        if self.lib == "oracle":
+            rc = getattr(oracle(), "orc_pb_decode_" + self.t)(C.c_void_p(self.h), level, ptrs, C.c_uint32(n),
+                                                              C.c_int32(exp), start, k,
+                                                              out.ctypes.data_as(C.c_void_p))
+        else:
+            rc = getattr(ref(), "ref_pb_decode_" + self.t)(C.c_void_p(self.h), level, ptrs, C.c_int32(n),
+                                                           C.c_int32(exp), start, k,
+                                                           out.ctypes.data_as(C.c_void_p))
+        assert rc == 0, rc
+        return out
+
+    def __del__(self):
+        try:
+            if self.lib == "oracle":
+                oracle().orc_pb_state_free(C.c_void_p(self.h))
+            else:
+                getattr(ref(), "ref_pb_free_" + self.t)(C.c_void_p(self.h))
+        except Exception:
+            pass
+
+
 # ---- collectors ---------------------------------------------------------------------
 def collect_max(bound, nb, dtype, lib="oracle"):
     t = _tp(dtype)

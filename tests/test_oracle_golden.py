@@ -78,6 +78,23 @@ def test_grouped(g, case):
     np.testing.assert_array_equal(got.view(np.uint8), g[f"{name}_full"].view(np.uint8))
 
 
+def test_perbit(g):
+    """PerBit streams and progressive decodes reproduce the reference's bytes"""
+    for i in range(int(g["pb_count"])):
+        x, e, nb = g[f"pb{i}_x"], int(g[f"pb{i}_exp"]), int(g[f"pb{i}_nb"])
+        sizes, raw = g[f"pb{i}_sizes"], g[f"pb{i}_bytes"].tobytes()
+        want, off = [], 0
+        for sz in sizes:
+            want.append(raw[off:off + int(sz)])
+            off += int(sz)
+        got = ob.pb_encode(x, e, nb)
+        assert got == want, i
+        dec = ob.PBDecoder(x.dtype)
+        for j, (s, k) in enumerate(splits(nb)[1:]):
+            np.testing.assert_array_equal(dec.decode(0, got[s:s + k], x.size, e, s, k).view(np.uint8),
+                                          g[f"pb{i}_dec{j}"].view(np.uint8))
+
+
 def test_collect_max(g):
     for t, dt, nb in (("f32", np.float32, 32), ("f64", np.float64, 64)):
         for b, want in zip(g[f"max_{t}_bounds"], g[f"max_{t}"]):

@@ -348,6 +348,49 @@ def test_squared_collector_is_exactly_rounded(nb):
     np.testing.assert_array_equal(got, np.array(want))
 
 
+# ---------------------------------------------------------------- PerBit ---
+
+
+def test_perbit_single_element_kat():
+    """-1.75 with exponent 1 and 4 planes: magnitude trunc(1.75 * 8) = 14 =
+    0b1110; stream 0 holds its leading 1 and then the sign 1 (bits LSB
+    first: 0b11), streams 1 and 2 a 1, stream 3 a 0; one word each."""
+    x = np.array([-1.75], np.float32)
+    s = ob.pb_encode(x, 1, 4)
+    words = [int(np.frombuffer(b, np.uint64)[0]) for b in s]
+    assert words == [0b11, 1, 1, 0]
+    assert ob.PBDecoder(np.float32).decode(0, s, 1, 1, 0, 4)[0] == np.float32(-1.75)
+
+
+@pytest.mark.parametrize("dt,nb", [(np.float32, 32), (np.float32, 12), (np.float64, 62), (np.float64, 30)])
+def test_perbit_stream_lengths_and_truncation(dt, nb):
+    """Stream p carries one bit per element plus one sign bit per element
+    whose leading 1 is bit nb-1-p (sizes = whole 64-bit words); the full
+    decode is sign * trunc(|x| 2^(nb-e)) 2^(e-nb); progressive deltas over
+    any split of the planes add up to it. (At 64 planes the reference's
+    int64 conversion saturates for |x| >= 2^(e-1), as in GroupedBP; that case
+    is pinned by the compiled reference instead.)"""
+    x = codec_x(900, dt, 3 + nb)
+    e = ob.frexp_exp(ob.maxabs(x), dt)
+    streams = ob.pb_encode(x, e, nb)
+    xs = x.astype(np.float64) * 2.0 ** (nb - e)
+    mag = np.trunc(np.abs(xs)).astype(np.uint64)
+    lead = np.full(x.size, -1)
+    nz = mag > 0
+    lead[nz] = nb - 1 - np.floor(np.log2(mag[nz].astype(np.float64))).astype(int)
+    for p in range(nb):
+        bits = x.size + int(np.sum(lead == p))
+        assert len(streams[p]) == (bits + 63) // 64 * 8, p
+    full = ob.PBDecoder(dt).decode(0, streams, x.size, e, 0, nb)
+    want = np.where(x < 0, -1.0, 1.0) * mag.astype(np.float64) * 2.0 ** (e - nb)
+    np.testing.assert_array_equal(full.astype(np.float64), want.astype(dt).astype(np.float64))
+    dec = ob.PBDecoder(dt)
+    acc = np.zeros(x.size, np.float64)
+    for s, t in ((0, 3), (3, nb // 2), (nb // 2, nb)):
+        acc += dec.decode(0, streams[s:t], x.size, e, s, t - s).astype(np.float64)
+    np.testing.assert_array_equal(acc, full.astype(np.float64))
+
+
 # ------------------------------------------------ encoder-fused level errors ---
 
 

@@ -91,6 +91,21 @@ def test_grouped(dt, nb, n, scale):
         np.testing.assert_array_equal(r1.view(np.uint8), r2.view(np.uint8))
 
 
+@pytest.mark.parametrize("dt,nb,n", [(np.float32, 32, 3000), (np.float32, 10, 64), (np.float64, 64, 2500),
+                                     (np.float64, 52, 7), (np.float32, 31, 1)])
+def test_perbit(dt, nb, n):
+    x = x_of(n, dt, n + nb)
+    e = ob.frexp_exp(ob.maxabs(x), dt)
+    a, b = ob.pb_encode(x, e, nb), ob.pb_encode(x, e, nb, lib="ref")
+    assert a == b
+    d1, d2 = ob.PBDecoder(dt), ob.PBDecoder(dt, lib="ref")
+    cuts = sorted({0, 1, nb // 3, nb // 2, nb})
+    for s, t in zip(cuts[:-1], cuts[1:]):
+        r1 = d1.decode(2, a[s:t], n, e, s, t - s)
+        r2 = d2.decode(0, b[s:t], n, e, s, t - s)
+        np.testing.assert_array_equal(r1.view(np.uint8), r2.view(np.uint8))
+
+
 @pytest.mark.parametrize("n,nb", [(2048, 32), (999, 24), (32, 2), (5000, 31)])
 @pytest.mark.parametrize("enc", ["nb", "gb"])
 def test_fused_level_errors_sequential(n, nb, enc):
