@@ -240,6 +240,15 @@ void ref_gb_level_errors_f32(const float *x, int32_t n, int32_t exp, int nb, dou
     std::copy(le.begin(), le.end(), out);
 }
 
+void ref_pb_level_errors_f32(const float *x, int32_t n, int32_t exp, int nb, double *out) {
+    MDR::PerBitBPEncoder<float, uint32_t> enc;
+    std::vector<uint32_t> sizes;
+    std::vector<double> le;
+    auto streams = enc.encode(x, n, exp, (uint8_t)nb, sizes, le);
+    for (auto *p : streams) free(p);
+    std::copy(le.begin(), le.end(), out);
+}
+
 void ref_collect_max_f32(float bound, int nb, double *out) {
     MDR::MaxErrorCollector<float> c;
     auto v = c.collect_level_error(nullptr, 0, nb, bound);

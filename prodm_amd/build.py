@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libprodm_mdr.so")
 SOURCES = ["capi.hip", "mgard.hip", "bitplane.hip", "interleave.hip", "collect.hip", "adjoint.hip", "batched.hip"]
-HOST_SOURCES = ["planner.cpp"]  # host-only C++ (g++)
+HOST_SOURCES = ["planner.cpp", "perbit_host.cpp"]  # host-only C++ (g++)
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -803,4 +803,266 @@ template <class T> size_t gb_decode_workspace_bytes(uint32_t n, int start, int k
 template size_t gb_decode_workspace_bytes<float>(uint32_t, int, int);
 template size_t gb_decode_workspace_bytes<double>(uint32_t, int, int);
 
+// ------------------------------------------------------------------------------
+// PerBit
+// ------------------------------------------------------------------------------
+// Stream p carries bit nb - 1 - p of every element's magnitude, element by
+// element, and right after an element's first 1 bit (over planes 0..nb-1)
+// its sign; bits fill 64-bit words LSB first (PerBitBPEncoder.hpp:9-39,
+// 96-143). Encode runs in three passes over workgroups of TPB lanes x B
+// consecutive elements: count (first-1 flags per plane), scan (bit offset
+// of every workgroup's run in every stream) and write (each lane expands
+// its plane word with its sign bits; the workgroup assembles the run in
+// LDS and stores whole words).
+
+// T_fp of the reference, sign ? -(int64_t) v : (int64_t) v with
+// v = ldexp(x, nb - exp) (PerBitBPEncoder.hpp:101-104). The conversion of
+// |v| >= 2^63 or NaN is undefined in C++; this gives the x86 result
+// (INT64_MIN, which is its own negation).
+template <class T>
+__device__ inline typename Traits<T>::Fixed pb_mag(T x, int shift) {
+    using F = typename Traits<T>::Fixed;
+    const T v = dev_ldexp(x, shift);
+    const int64_t q = fabs(v) < (T)9.2233720368547758e18 ? (int64_t)v : INT64_MIN;
+    return (F)(x < 0 ? (uint64_t)0 - (uint64_t)q : (uint64_t)q);
+}
+
+template <class T> struct PbRun;  // a lane's run: up to 2 B bits
+template <> struct PbRun<float> { using R = uint64_t; };
+template <> struct PbRun<double> { using R = unsigned __int128; };
+
+// sum over the wave of c (0 <= c < 128), by bit slices
+__device__ inline uint32_t pb_wave_sum(uint32_t c) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int b = 0; b < 7; b++) s += (uint32_t)__popcll(ballot64((c >> b) & 1u)) << b;
+    return s;
+}
+// sum of c over the lower lanes of the wave
+__device__ inline uint32_t pb_wave_prefix(uint32_t c) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int b = 0; b < 7; b++) s += lanes_below(ballot64((c >> b) & 1u)) << b;
+    return s;
+}
+
+// the lane's magnitudes as plane words: D[k] bit j = bit k of element j
+template <class T> struct PbPlanes;
+template <> struct PbPlanes<float> {
+    uint32_t m[32];
+    uint32_t sg;
+    __device__ void load(const float *row, int shift) {
+        sg = 0;
+#pragma unroll
+        for (int j = 0; j < 32; j++) {
+            m[j] = pb_mag<float>(row[j], shift);
+            sg |= (uint32_t)(row[j] < 0) << j;
+        }
+        transpose32(m);
+    }
+    template <int K> __device__ uint64_t word() const { return m[K]; }
+};
+template <> struct PbPlanes<double> {
+    uint32_t lo[64], hi[64];
+    uint64_t sg;
+    __device__ void load(const double *row, int shift) {
+        sg = 0;
+#pragma unroll
+        for (int j = 0; j < 64; j++) {
+            const uint64_t q = pb_mag<double>(row[j], shift);
+            lo[j] = (uint32_t)q;
+            hi[j] = (uint32_t)(q >> 32);
+            sg |= (uint64_t)(row[j] < 0) << j;
+        }
+        transpose64(lo, hi);
+    }
+    template <int K> __device__ uint64_t word() const { return word64_get<K>(lo, hi); }
+};
+
+// calls f(p, D, F) for the planes p = 0..nb-1 in order (D = the plane word,
+// F = the elements whose first 1 bit is in plane p)
+template <class T, class Fn> __device__ __forceinline__ void pb_for_planes(const PbPlanes<T> &P, int nb, Fn &&f) {
+    constexpr int B = BlkStage<T>::kB;
+    uint64_t H = 0;
+    static_for<0, B>([&](auto kc) {
+        constexpr int k = B - 1 - decltype(kc)::value;
+        const int p = nb - 1 - k;
+        if (p >= 0) {
+            const uint64_t D = P.template word<k>();
+            const uint64_t F = D & ~H;
+            H |= D;
+            f(p, D, F);
+        }
+    });
+}
+
+// Pass 1: bits per plane of each workgroup's run: wg_bits[wg][p] =
+// (valid elements) + (elements whose first 1 is in plane p).
+template <class T>
+__global__ __launch_bounds__(BlkStage<T>::kTPB) void k_pb_count(const T *__restrict__ x, uint32_t n, int32_t exp,
+                                                             const KeyOf<T> *key, int nb,
+                                                             uint32_t *__restrict__ wg_bits) {
+    constexpr int TPB = BlkStage<T>::kTPB, B = BlkStage<T>::kB, PI = BlkStage<T>::kPitch;
+    __shared__ T sx[TPB * PI];
+    __shared__ uint32_t cnt[64];
+    for (int i = threadIdx.x; i < nb; i += TPB) cnt[i] = 0;
+    const uint64_t e0 = (uint64_t)blockIdx.x * TPB * B;
+    stage_in<T>(x, n, e0, sx);
+    __syncthreads();
+    PbPlanes<T> P;
+    P.load(sx + threadIdx.x * PI, nb - level_exp<T>(exp, key));
+    pb_for_planes<T>(P, nb, [&](int p, uint64_t, uint64_t F) {
+        const uint32_t s = pb_wave_sum((uint32_t)__popcll(F));
+        if (lane_id() == 0 && s) atomicAdd(&cnt[p], s);
+    });
+    __syncthreads();
+    const uint32_t nv = (uint32_t)min((uint64_t)TPB * B, (uint64_t)n - e0);
+    for (int i = threadIdx.x; i < nb; i += TPB) wg_bits[(uint64_t)blockIdx.x * nb + i] = nv + cnt[i];
+}
+
+// Pass 2, one workgroup per plane: exclusive bit offsets off[wg][p] of the
+// workgroup runs, the stream size in bytes (whole words), and zeros in
+// the words that two runs share (they are completed by atomic ORs).
+__global__ __launch_bounds__(1024) void k_pb_scan(const uint32_t *__restrict__ wg_bits, uint32_t nwg, int nb,
+                                                  uint64_t *__restrict__ off, uint64_t *__restrict__ streams,
+                                                  uint64_t stride, uint32_t *__restrict__ sizes) {
+    __shared__ uint32_t sh[1024];
+    const int p = blockIdx.x;
+    uint64_t *st = streams + (uint64_t)p * stride;
+    uint64_t carry = 0;
+    for (uint32_t c0 = 0; c0 < nwg; c0 += 1024) {
+        const uint32_t wg = c0 + threadIdx.x;
+        const uint32_t v = wg < nwg ? wg_bits[(uint64_t)wg * nb + p] : 0;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            const uint32_t t = threadIdx.x >= (uint32_t)o ? sh[threadIdx.x - o] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (wg < nwg) {
+            const uint64_t o = carry + sh[threadIdx.x] - v;
+            off[(uint64_t)wg * nb + p] = o;
+            if (o & 63) st[o >> 6] = 0;
+        }
+        carry += sh[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (carry & 63) st[carry >> 6] = 0;
+        sizes[p] = (uint32_t)(((carry + 63) >> 6) * 8);
+    }
+}
+
+// Pass 3: the streams. Lane t's run in plane p starts at bit
+// off[wg][p] + min(t B, valid) + (first-1 flags of the lanes below).
+template <class T>
+__global__ __launch_bounds__(BlkStage<T>::kTPB) void k_pb_write(const T *__restrict__ x, uint32_t n, int32_t exp,
+                                                             const KeyOf<T> *key, int nb,
+                                                             const uint64_t *__restrict__ off,
+                                                             uint64_t *__restrict__ streams, uint64_t stride) {
+    using R = typename PbRun<T>::R;
+    constexpr int TPB = BlkStage<T>::kTPB, B = BlkStage<T>::kB, PI = BlkStage<T>::kPitch, NW = TPB / 64;
+    constexpr int kRunWords = (2 * TPB * B + 63) / 64 + 2;  // a workgroup run plus its start offset
+    __shared__ T sx[TPB * PI];
+    __shared__ unsigned long long run[2][kRunWords];
+    __shared__ uint32_t wt[64][NW];  // first-1 flags per plane and wave
+    for (int i = threadIdx.x; i < 2 * kRunWords; i += TPB) (&run[0][0])[i] = 0;
+    const uint64_t e0 = (uint64_t)blockIdx.x * TPB * B;
+    stage_in<T>(x, n, e0, sx);
+    __syncthreads();
+    PbPlanes<T> P;
+    P.load(sx + threadIdx.x * PI, nb - level_exp<T>(exp, key));
+    const uint32_t wv = threadIdx.x / 64;
+    pb_for_planes<T>(P, nb, [&](int p, uint64_t, uint64_t F) {
+        const uint32_t s = pb_wave_sum((uint32_t)__popcll(F));
+        if (lane_id() == 0) wt[p][wv] = s;
+    });
+    __syncthreads();
+    const uint32_t nv = (uint32_t)min((uint64_t)TPB * B, (uint64_t)n - e0);
+    const uint32_t below = min((uint32_t)threadIdx.x * B, nv);  // valid elements of the lanes below
+    const uint64_t S = P.sg;
+    pb_for_planes<T>(P, nb, [&](int p, uint64_t D, uint64_t F) {
+        unsigned long long *buf = run[p & 1];
+        const uint64_t o = off[(uint64_t)blockIdx.x * nb + p];
+        uint32_t flags = 0, wbelow = 0;
+        for (int w = 0; w < NW; w++) {
+            flags += wt[p][w];
+            if (w < (int)wv) wbelow += wt[p][w];
+        }
+        const uint32_t q = (uint32_t)(o & 63) + below + wbelow + pb_wave_prefix((uint32_t)__popcll(F));
+        // insert each flagged element's sign after its bit, highest first
+        R r = (R)D;
+        uint64_t ff = F;
+        while (ff) {
+            const int j = 63 - __clzll(ff);
+            ff ^= 1ull << j;
+            const R low = ((R)2 << j) - 1;
+            r = (r & low) | ((R)((S >> j) & 1u) << (j + 1)) | ((r & ~low) << 1);
+        }
+        const uint32_t wi = q >> 6, sh = q & 63;
+        if constexpr (sizeof(T) == 4) {
+            const uint64_t w0 = r << sh, w1 = sh ? r >> (64 - sh) : 0;
+            if (w0) atomicOr(&buf[wi], w0);
+            if (w1) atomicOr(&buf[wi + 1], w1);
+        } else {
+            const uint64_t rl = (uint64_t)r, rh = (uint64_t)(r >> 64);
+            const uint64_t w0 = rl << sh;
+            const uint64_t w1 = sh ? (rh << sh) | (rl >> (64 - sh)) : rh;
+            const uint64_t w2 = sh ? rh >> (64 - sh) : 0;
+            if (w0) atomicOr(&buf[wi], w0);
+            if (w1) atomicOr(&buf[wi + 1], w1);
+            if (w2) atomicOr(&buf[wi + 2], w2);
+        }
+        __syncthreads();
+        // whole words of this run are stored; the first and last may be
+        // shared with the neighbouring runs (zeroed by the scan) and are ORed
+        const uint64_t end = o + nv + flags;
+        const uint32_t nwords = (uint32_t)(((o & 63) + nv + flags + 63) >> 6);
+        uint64_t *st = streams + (uint64_t)p * stride + (o >> 6);
+        for (uint32_t w = threadIdx.x; w < nwords; w += TPB) {
+            const uint64_t v = buf[w];
+            buf[w] = 0;
+            const uint64_t gb = ((o >> 6) + w) << 6;  // first bit of the word
+            if (gb >= o && gb + 64 <= end) st[w] = v;
+            else if (v) atomicOr((unsigned long long *)&st[w], (unsigned long long)v);
+        }
+    });
+}
+
+template <class T> static uint32_t pb_nwg(uint32_t n) { return div_up(n, (uint64_t)BlkStage<T>::kTPB * BlkStage<T>::kB); }
+
+uint64_t pb_stream_words(uint32_t n) { return (2ull * n + 63) / 64; }
+
+template <class T> size_t pb_workspace_bytes(uint32_t n, int nb) {
+    const uint64_t nwg = pb_nwg<T>(n);
+    return nwg * nb * 8 + nwg * nb * 4 + 256;
+}
+template size_t pb_workspace_bytes<float>(uint32_t, int);
+template size_t pb_workspace_bytes<double>(uint32_t, int);
+
+template <class T>
+int launch_pb_encode(const T *x, uint32_t n, int32_t exp, const KeyOf<T> *key, int nb, uint64_t *streams,
+                     uint64_t stride, uint32_t *sizes, void *ws, size_t ws_bytes, hipStream_t s) {
+    if (n == 0 || nb < 1 || nb > (int)(8 * sizeof(T))) { set_error("perbit encode: bad n or num_bitplanes"); return kErrArg; }
+    if (stride < pb_stream_words(n)) { set_error("perbit encode: stream stride below ceil(2 n / 64) words"); return kErrArg; }
+    if (ws_bytes < pb_workspace_bytes<T>(n, nb)) { set_error("perbit encode: workspace too small"); return kErrArg; }
+    const uint32_t nwg = pb_nwg<T>(n);
+    uint64_t *off = static_cast<uint64_t *>(ws);
+    uint32_t *wg_bits = reinterpret_cast<uint32_t *>(off + (size_t)nwg * nb);
+    hipLaunchKernelGGL(k_pb_count<T>, dim3(nwg), dim3(BlkStage<T>::kTPB), 0, s, x, n, exp, key, nb, wg_bits);
+    PRODM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_pb_scan, dim3(nb), dim3(1024), 0, s, wg_bits, nwg, nb, off, streams, stride, sizes);
+    PRODM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_pb_write<T>, dim3(nwg), dim3(BlkStage<T>::kTPB), 0, s, x, n, exp, key, nb, off, streams,
+                       stride);
+    PRODM_CHECK_LAUNCH();
+    return kOk;
+}
+template int launch_pb_encode<float>(const float *, uint32_t, int32_t, const uint32_t *, int, uint64_t *, uint64_t,
+                                     uint32_t *, void *, size_t, hipStream_t);
+template int launch_pb_encode<double>(const double *, uint32_t, int32_t, const unsigned long long *, int, uint64_t *,
+                                      uint64_t, uint32_t *, void *, size_t, hipStream_t);
+
 }  // namespace prodm

@@ -588,6 +588,21 @@ int prodm_grouped_decode_f64(prodm_grouped_state *state, int level, const uint8_
     return grouped_decode_impl<double>(state, level, h_stream_ptrs, n, exp, start, k, d_out, (hipStream_t)stream);
 }
 
+uint64_t prodm_perbit_stream_words(uint32_t n) { return pb_stream_words(n); }
+size_t prodm_perbit_workspace_bytes(uint32_t n, int nb, int dtype) {
+    return dtype == PRODM_F64 ? pb_workspace_bytes<double>(n, nb) : pb_workspace_bytes<float>(n, nb);
+}
+int prodm_perbit_encode_f32(const float *d_x, uint32_t n, int32_t exp, int nb, uint64_t *d_streams,
+                            uint64_t stride, uint32_t *d_sizes, void *d_ws, size_t ws_bytes, prodm_stream_t stream) {
+    return launch_pb_encode<float>(d_x, n, exp, nullptr, nb, d_streams, stride, d_sizes, d_ws, ws_bytes,
+                                   (hipStream_t)stream);
+}
+int prodm_perbit_encode_f64(const double *d_x, uint32_t n, int32_t exp, int nb, uint64_t *d_streams,
+                            uint64_t stride, uint32_t *d_sizes, void *d_ws, size_t ws_bytes, prodm_stream_t stream) {
+    return launch_pb_encode<double>(d_x, n, exp, nullptr, nb, d_streams, stride, d_sizes, d_ws, ws_bytes,
+                                    (hipStream_t)stream);
+}
+
 void prodm_collect_max_f32(float bound, int nb, double *h_out) { collect_max_impl<float>(bound, nb, h_out); }
 void prodm_collect_max_f64(double bound, int nb, double *h_out) { collect_max_impl<double>(bound, nb, h_out); }
 

@@ -72,6 +72,15 @@ int launch_gb_decode(const PlanePtrs &streams, uint32_t n, int32_t exp, const Ke
                      uint64_t hdr0_bytes, hipStream_t s);
 template <class T> size_t gb_decode_workspace_bytes(uint32_t n, int start, int k);
 
+// ---- PerBit (PerBitBPEncoder.hpp:81-143) -------------------------------------
+// Stream p (64-bit words) at streams + p * stride, stride >= pb_stream_words(n);
+// sizes[p] (device) = its bytes, whole words.
+uint64_t pb_stream_words(uint32_t n);
+template <class T> size_t pb_workspace_bytes(uint32_t n, int nb);
+template <class T>
+int launch_pb_encode(const T *x, uint32_t n, int32_t exp, const KeyOf<T> *key, int nb, uint64_t *streams,
+                     uint64_t stride, uint32_t *sizes, void *ws, size_t ws_bytes, hipStream_t s);
+
 // ---- interleave (DirectInterleaver.hpp:12-63) --------------------------------
 template <class T>
 int launch_interleave(const T *data, const uint64_t *strides3, const uint32_t *fine3, const uint32_t *coarse3, T *buf,

@@ -207,6 +207,61 @@ class GroupedDecoder:
             pass
 
 
+# ---- PerBit ---------------------------------------------------------------------------
+def perbit_encode(x, exp, nb):
+    """Returns (streams uint64-word tensor (nb, stride), sizes int32 tensor of
+    bytes); stream p is row p[:sizes[p] // 8] (PerBitBPEncoder.hpp:81-143)."""
+    n = x.numel()
+    stride = int(lib().prodm_perbit_stream_words(n))
+    streams = torch.empty((nb, max(stride, 1)), dtype=torch.int64, device=x.device)
+    sizes = torch.zeros(nb, dtype=torch.int32, device=x.device)
+    ws = _ws(lib().prodm_perbit_workspace_bytes(n, nb, _TDT[x.dtype]))
+    check(_fn("prodm_perbit_encode", x.dtype)(_ptr(x), n, int(exp), nb, _ptr(streams), stride, _ptr(sizes), _ptr(ws),
+                                              ws.numel(), _stream()), "perbit encode")
+    return streams, sizes
+
+
+def perbit_streams(streams, sizes):
+    """Device encode output -> list of the reference's byte streams."""
+    sh = streams.cpu().numpy().view(np.uint64)
+    zh = sizes.cpu().numpy()
+    return [sh[p, : int(zh[p]) // 8].tobytes() for p in range(sh.shape[0])]
+
+
+class PerBitDecoder:
+    """Progressive PerBit decode (PerBitBPEncoder.hpp:290-382). The format
+    has no index of where an element's bits begin, so it decodes on the
+    host; the per-level sign state is kept in the library, keyed by level."""
+
+    def __init__(self, dtype):
+        self.dtype = dtype
+        h = C.c_void_p()
+        check(lib().prodm_perbit_state_create(C.byref(h)), "perbit state")
+        self.h = h
+
+    def decode(self, level, streams, n, exp, start, k):
+        """streams: list of byte strings (stream start + i); returns numpy."""
+        npdt = np.float32 if self.dtype == torch.float32 else np.float64
+        out = np.empty(n, dtype=npdt)
+        bufs = [np.frombuffer(s, dtype=np.uint64) if len(s) else np.zeros(1, np.uint64) for s in streams[:k]]
+        ptrs = (C.c_void_p * max(k, 1))(*[b.ctypes.data for b in bufs])
+        sizes = (C.c_uint32 * max(k, 1))(*[len(s) for s in streams[:k]])
+        check(_fn("prodm_perbit_decode", self.dtype)(self.h, level, ptrs, sizes, n, int(exp), start, k,
+                                                     out.ctypes.data), "perbit decode")
+        return out
+
+    def close(self):
+        if self.h:
+            lib().prodm_perbit_state_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---- collectors ---------------------------------------------------------------------------
 def collect_max(bound, nb, dtype=torch.float32):
     out = (C.c_double * (nb + 1))()

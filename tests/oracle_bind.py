@@ -316,7 +316,8 @@ def collect_squared(x, nb, maxerr, mode="exact"):
 
 def fused_level_errors(x, exp, nb, enc, mode="exact"):
     """Level-error table of the encoders' level_errors overloads (fp32):
-    enc "nb" (NegaBinaryBPEncoder) or "gb" (GroupedBPEncoder); mode "exact"
+    enc "nb" (NegaBinaryBPEncoder), "gb" (GroupedBPEncoder) or, with mode "ref"
+    only, "pb" (PerBitBPEncoder); mode "exact"
     (exactly rounded sums), "seq" (reference order) or "ref" (the compiled
     reference, n >= 32)."""
     assert x.dtype == np.float32

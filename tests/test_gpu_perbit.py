@@ -1,0 +1,109 @@
+"""GPU PerBit encode (prodm_perbit_encode_*) against the oracle: the
+streams and their sizes are byte-identical, for f32 / f64, one to many
+workgroups, ragged tails, special values (zeros, -0, NaN, data above
+2^exp), and the host decoder reconstructs the truncated fixed point from
+the GPU's streams; PerBit level errors are GroupedBP's."""
+import numpy as np
+import pytest
+import torch
+
+import oracle_bind as ob
+from prodm_amd import capi, mdr
+
+pytestmark = pytest.mark.gpu
+
+TD = {np.float32: torch.float32, np.float64: torch.float64}
+
+
+def field(n, dt, seed):
+    rng = np.random.default_rng(seed)
+    x = (rng.standard_normal(n) * np.exp(rng.uniform(-14, 0, n))).astype(dt)
+    x[rng.random(n) < 0.05] = 0
+    return x
+
+
+def level_exp(x):
+    return ob.frexp_exp(ob.maxabs(x), x.dtype.type)
+
+
+def encode_gpu(x, e, nb):
+    streams, sizes = mdr.perbit_encode(torch.from_numpy(x).cuda(), e, nb)
+    torch.cuda.synchronize()
+    return mdr.perbit_streams(streams, sizes), sizes.cpu().numpy()
+
+
+@pytest.mark.parametrize("dt,nb,n", [
+    (np.float32, 32, 1), (np.float32, 32, 31), (np.float32, 32, 33), (np.float32, 32, 8191),
+    (np.float32, 32, 8193), (np.float32, 17, 100003), (np.float32, 1, 5000), (np.float32, 32, 1 << 20),
+    (np.float64, 64, 1), (np.float64, 64, 65), (np.float64, 64, 4097), (np.float64, 52, 100003),
+    (np.float64, 3, 777), (np.float64, 64, 300000),
+])
+def test_perbit_encode_matches_oracle(gpu, dt, nb, n):
+    x = field(n, dt, n + nb)
+    e = level_exp(x)
+    mine, sizes = encode_gpu(x, e, nb)
+    ref = ob.pb_encode(x, e, nb)
+    assert [len(r) for r in ref] == list(sizes)
+    for p in range(nb):
+        assert mine[p] == ref[p], (p, n, nb)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_perbit_special_values(gpu, dt):
+    x = field(20000, dt, 4)
+    x[:8] = [0.0, -0.0, np.nan, -np.nan, 1e-38, -1e-38, 0.5, -0.75]
+    x[100:140:3] = -x[100:140:3]
+    e = level_exp(np.nan_to_num(x))
+    for nb in (8 * x.itemsize, 9):
+        mine, _ = encode_gpu(x, e, nb)
+        assert mine == ob.pb_encode(x, e, nb), nb
+    # data above 2^exp: the fixed point keeps only its low nb bits; at
+    # |v| >= 2^63 the conversion gives the x86 result
+    y = x.copy()
+    y[10:20] = [3.0, -5.0, 1e3, -1e6, 2.0 ** 40, -(2.0 ** 62), 1e30, -1e30, np.inf, -np.inf]
+    for nb in (8 * x.itemsize, 20):
+        mine, _ = encode_gpu(y, 0, nb)
+        assert mine == ob.pb_encode(y, 0, nb), nb
+
+
+def test_perbit_roundtrip_large(gpu):
+    """8 M elements: the host decoder reconstructs trunc(|x| 2^(nb-e)) with
+    the signs from the GPU's streams (planes decoded in three steps)."""
+    n, nb = 8 << 20, 32
+    rng = np.random.default_rng(11)
+    x = (rng.standard_normal(n) * 1e-3).astype(np.float32)
+    e = level_exp(x)
+    streams, _ = encode_gpu(x, e, nb)
+    d = mdr.PerBitDecoder(torch.float32)
+    acc = np.zeros(n, np.float64)
+    for s, k in ((0, 10), (10, 12), (22, 10)):
+        acc += d.decode(0, streams[s:s + k], n, e, s, k)
+    t = np.trunc(np.abs(x).astype(np.float64) * 2.0 ** (nb - e)) * 2.0 ** (e - nb)
+    assert np.array_equal(np.abs(acc), t)
+    nz = t != 0
+    assert np.array_equal(np.signbit(acc[nz]), np.signbit(x[nz]))
+    # every stream holds n bits plus one sign per element whose first 1 it has
+    assert sum(len(s) for s in streams) >= nb * n // 8
+
+
+def test_perbit_level_errors_are_grouped(gpu):
+    x = field(50000, np.float32, 8)
+    e = level_exp(x)
+    got = mdr.level_errors(torch.from_numpy(x).cuda(), e, 32, encoder=capi.GROUPED)
+    np.testing.assert_array_equal(got, ob.fused_level_errors(x, e, 32, "gb"))
+
+
+def test_perbit_encode_rejects_bad_arguments(gpu):
+    lib = capi.lib()
+    x = torch.zeros(100, device="cuda")
+    out = torch.zeros((33, 8), dtype=torch.int64, device="cuda")
+    sizes = torch.zeros(33, dtype=torch.int32, device="cuda")
+    ws = torch.zeros(1 << 16, dtype=torch.uint8, device="cuda")
+    s = mdr._stream()
+    args = lambda n, nb, stride: (mdr._ptr(x), n, 0, nb, mdr._ptr(out), stride, mdr._ptr(sizes), mdr._ptr(ws),
+                                  ws.numel(), s)
+    assert lib.prodm_perbit_encode_f32(*args(0, 8, 8)) != 0        # no elements
+    assert lib.prodm_perbit_encode_f32(*args(100, 33, 8)) != 0     # more planes than bits
+    assert lib.prodm_perbit_encode_f32(*args(100, 8, 3)) != 0      # stride below ceil(2 n / 64)
+    assert lib.prodm_perbit_encode_f32(*args(100, 8, 4)) == 0
+    torch.cuda.synchronize()

@@ -115,6 +115,16 @@ def test_fused_level_errors_sequential(n, nb, enc):
                                   ob.fused_level_errors(x, e, nb, enc, mode="ref"))
 
 
+
+@pytest.mark.parametrize("n,nb", [(2048, 32), (999, 24), (5000, 31)])
+def test_perbit_level_errors_are_groupedbp_level_errors(n, nb):
+    """PerBitBPEncoder's level-error overload has GroupedBP's per-element
+    terms in the same element order: the reference's two tables agree."""
+    x = x_of(n, np.float32, 3 * n + nb)
+    e = ob.frexp_exp(ob.maxabs(x), np.float32)
+    np.testing.assert_array_equal(ob.fused_level_errors(x, e, nb, "pb", mode="ref"),
+                                  ob.fused_level_errors(x, e, nb, "gb", mode="ref"))
+
 @pytest.mark.parametrize("n,nb", [(1000, 32), (777, 16), (64, 30)])
 def test_squared_sequential(n, nb):
     x = x_of(n, np.float32, n)
