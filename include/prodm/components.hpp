@@ -10,6 +10,7 @@
 //   GpuDirectInterleaver<T>          MDR::DirectInterleaver<T>          (Interleaver/DirectInterleaver.hpp:9-67)
 //   GpuNegaBinaryBPEncoder<T, S>     MDR::NegaBinaryBPEncoder<T, S>     (BitplaneEncoder/NegaBinaryBPEncoder.hpp:12-262)
 //   GpuGroupedBPEncoder<T, S>        MDR::GroupedBPEncoder<T, S>        (BitplaneEncoder/GroupedBPEncoder.hpp:12-380)
+//   GpuPerBitBPEncoder<T, S>         MDR::PerBitBPEncoder<T, S>         (BitplaneEncoder/PerBitBPEncoder.hpp:70-403)
 //   MaxErrorCollector<T>             MDR::MaxErrorCollector<T>          (ErrorCollector/MaxErrorCollector.hpp:9-30)
 //   GpuSquaredErrorCollector<T>      MDR::SquaredErrorCollector<T>      (ErrorCollector/SquaredErrorCollector.hpp:14-70)
 //
@@ -354,6 +355,91 @@ class GpuGroupedBPEncoder {
         }
     };
     static uint64_t blocks(int32_t n) { return n > 0 ? ((uint64_t)n - 1) / (8 * sizeof(S)) + 1 : 0; }
+    Context ctx_;
+    std::shared_ptr<State> st_;
+    int compat_;
+};
+
+// ---------------------------------------------------------------- per-bit
+// Encode on the GPU; progressive_decode on the host (the streams carry no
+// index of where an element's bits begin, so each is parsed from its
+// start), with the per-level sign state keyed by `level`.
+template <class T, class S>
+class GpuPerBitBPEncoder {
+  public:
+    explicit GpuPerBitBPEncoder(Context ctx = make_context(), bool x86_compat = false)
+        : ctx_(std::move(ctx)), st_(std::make_shared<State>()), compat_(x86_compat) {
+        detail::check_stream_type<T, S>();
+    }
+    std::vector<uint8_t *> encode(T const *data, int32_t n, int32_t exp, uint8_t num_bitplanes,
+                                  std::vector<uint32_t> &stream_sizes) const {
+        DeviceContext &c = *ctx_;
+        const uint64_t stride = prodm_perbit_stream_words((uint32_t)n);
+        T *x = c.a.as<T>(n);
+        uint64_t *planes = c.b.as<uint64_t>(stride * num_bitplanes);
+        uint32_t *sizes = c.aux.as<uint32_t>(num_bitplanes);
+        const size_t wsb = prodm_perbit_workspace_bytes((uint32_t)n, num_bitplanes, dtype_of<T>());
+        void *ws = c.ws.get(wsb);
+        to_device(x, data, (uint64_t)n, c.stream);
+        int rc;
+        if constexpr (std::is_same<T, float>::value)
+            rc = prodm_perbit_encode_f32(x, (uint32_t)n, exp, num_bitplanes, compat_, planes, stride, sizes, ws, wsb,
+                                         c.stream);
+        else
+            rc = prodm_perbit_encode_f64(x, (uint32_t)n, exp, num_bitplanes, compat_, planes, stride, sizes, ws, wsb,
+                                         c.stream);
+        check(rc, "perbit encode");
+        stream_sizes.assign(num_bitplanes, 0);
+        to_host(stream_sizes.data(), sizes, num_bitplanes, c.stream);
+        std::vector<uint8_t *> streams(num_bitplanes);
+        for (int p = 0; p < num_bitplanes; p++) {
+            streams[p] = static_cast<uint8_t *>(std::malloc(stream_sizes[p] + 8));
+            to_host(streams[p], reinterpret_cast<const uint8_t *>(planes + p * stride), stream_sizes[p], c.stream);
+        }
+        return streams;
+    }
+    // encode with the per-plane level errors (PerBitBPEncoder.hpp:151-226):
+    // the per-element terms are GroupedBP's (:387-397), fp32 only.
+    std::vector<uint8_t *> encode(T const *data, int32_t n, int32_t exp, uint8_t num_bitplanes,
+                                  std::vector<uint32_t> &stream_sizes, std::vector<double> &level_errors) const {
+        static_assert(std::is_same<T, float>::value, "the encoder's level errors are defined for float data");
+        std::vector<uint8_t *> streams = encode(data, n, exp, num_bitplanes, stream_sizes);
+        DeviceContext &c = *ctx_;
+        const size_t wsb = prodm_level_errors_workspace_bytes((uint64_t)n, num_bitplanes);
+        level_errors.assign(num_bitplanes + 1, 0.0);
+        check(prodm_grouped_level_errors_f32(c.a.as<float>(n), (uint64_t)n, exp, num_bitplanes, level_errors.data(),
+                                             c.ws.get(wsb), wsb, c.stream),
+              "perbit level errors");
+        return streams;
+    }
+    T *decode(const std::vector<uint8_t const *> &streams, int32_t n, int exp, uint8_t num_bitplanes) {
+        GpuPerBitBPEncoder fresh(ctx_, compat_);
+        return fresh.progressive_decode(streams, n, exp, 0, num_bitplanes, 0);
+    }
+    T *progressive_decode(const std::vector<uint8_t const *> &streams, int32_t n, int exp, uint8_t start, uint8_t k,
+                          int level) {
+        State &s = *st_;
+        if (!s.h) check(prodm_perbit_state_create(&s.h), "perbit state");
+        std::vector<const uint64_t *> ptrs(k);
+        for (int i = 0; i < k; i++) ptrs[i] = reinterpret_cast<const uint64_t *>(streams[i]);
+        T *h = static_cast<T *>(std::malloc(sizeof(T) * (n ? n : 1)));
+        int rc;
+        if constexpr (std::is_same<T, float>::value)
+            rc = prodm_perbit_decode_f32(s.h, level, ptrs.data(), nullptr, (uint32_t)n, exp, start, k, h);
+        else
+            rc = prodm_perbit_decode_f64(s.h, level, ptrs.data(), nullptr, (uint32_t)n, exp, start, k, h);
+        check(rc, "perbit decode");
+        return h;
+    }
+    void print() const { std::printf("per-bit bitplane encoder (MI355X encode, host decode)\n"); }
+
+  private:
+    struct State {
+        prodm_perbit_state *h = nullptr;
+        ~State() {
+            if (h) prodm_perbit_state_destroy(h);
+        }
+    };
     Context ctx_;
     std::shared_ptr<State> st_;
     int compat_;

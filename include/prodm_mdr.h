@@ -152,7 +152,8 @@ int prodm_grouped_decode_f64(prodm_grouped_state *state, int level, const uint8_
  *      its sign, in 64-bit words filled LSB first. Encode (device) writes
  *      stream p at d_streams + p * stride_words (stride_words >=
  *      prodm_perbit_stream_words(n)) and d_sizes[p] = its bytes (whole
- *      words), as the reference's stream_sizes. Decode runs on the host
+ *      words), as the reference's stream_sizes; compat as for GroupedBP
+ *      (x86's (int64_t) on |v| >= 2^63 or NaN). Decode runs on the host
  *      (the format has no index of where an element's bits begin):
  *      h_streams[i] is stream start + i in host memory, h_sizes[i] its
  *      bytes (h_sizes may be NULL: no bound is then checked); the
@@ -161,12 +162,12 @@ int prodm_grouped_decode_f64(prodm_grouped_state *state, int level, const uint8_
  *      GroupedBP (same per-element terms): prodm_grouped_level_errors_f32. */
 uint64_t prodm_perbit_stream_words(uint32_t n);
 size_t prodm_perbit_workspace_bytes(uint32_t n, int num_bitplanes, int dtype);
-int prodm_perbit_encode_f32(const float *d_x, uint32_t n, int32_t exp, int num_bitplanes, uint64_t *d_streams,
-                            uint64_t stride_words, uint32_t *d_sizes, void *d_ws, size_t ws_bytes,
-                            prodm_stream_t stream);
-int prodm_perbit_encode_f64(const double *d_x, uint32_t n, int32_t exp, int num_bitplanes, uint64_t *d_streams,
-                            uint64_t stride_words, uint32_t *d_sizes, void *d_ws, size_t ws_bytes,
-                            prodm_stream_t stream);
+int prodm_perbit_encode_f32(const float *d_x, uint32_t n, int32_t exp, int num_bitplanes, int compat,
+                            uint64_t *d_streams, uint64_t stride_words, uint32_t *d_sizes, void *d_ws,
+                            size_t ws_bytes, prodm_stream_t stream);
+int prodm_perbit_encode_f64(const double *d_x, uint32_t n, int32_t exp, int num_bitplanes, int compat,
+                            uint64_t *d_streams, uint64_t stride_words, uint32_t *d_sizes, void *d_ws,
+                            size_t ws_bytes, prodm_stream_t stream);
 typedef struct prodm_perbit_state prodm_perbit_state;
 int prodm_perbit_state_create(prodm_perbit_state **out);
 int prodm_perbit_state_destroy(prodm_perbit_state *state);

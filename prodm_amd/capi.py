@@ -56,8 +56,8 @@ _SIGS = {
     "prodm_grouped_decode_f64": (C.c_int, [vp, C.c_int, vp, u32, i32, C.c_int, C.c_int, vp, vp]),
     "prodm_perbit_stream_words": (u64, [u32]),
     "prodm_perbit_workspace_bytes": (sz, [u32, C.c_int, C.c_int]),
-    "prodm_perbit_encode_f32": (C.c_int, [vp, u32, i32, C.c_int, vp, u64, vp, vp, sz, vp]),
-    "prodm_perbit_encode_f64": (C.c_int, [vp, u32, i32, C.c_int, vp, u64, vp, vp, sz, vp]),
+    "prodm_perbit_encode_f32": (C.c_int, [vp, u32, i32, C.c_int, C.c_int, vp, u64, vp, vp, sz, vp]),
+    "prodm_perbit_encode_f64": (C.c_int, [vp, u32, i32, C.c_int, C.c_int, vp, u64, vp, vp, sz, vp]),
     "prodm_perbit_state_create": (C.c_int, [vp]),
     "prodm_perbit_state_destroy": (C.c_int, [vp]),
     "prodm_perbit_decode_f32": (C.c_int, [vp, C.c_int, vp, vp, u32, i32, C.c_int, C.c_int, vp]),

@@ -817,11 +817,13 @@ template size_t gb_decode_workspace_bytes<double>(uint32_t, int, int);
 
 // T_fp of the reference, sign ? -(int64_t) v : (int64_t) v with
 // v = ldexp(x, nb - exp) (PerBitBPEncoder.hpp:101-104). The conversion of
-// |v| >= 2^63 or NaN is undefined in C++; this gives the x86 result
-// (INT64_MIN, which is its own negation).
+// |v| >= 2^63 or NaN is undefined in C++: compat gives the x86 result
+// (INT64_MIN, which is its own negation), otherwise the truncated
+// magnitude is used, as for GroupedBP (hazard B1).
 template <class T>
-__device__ inline typename Traits<T>::Fixed pb_mag(T x, int shift) {
+__device__ inline typename Traits<T>::Fixed pb_mag(T x, int shift, int compat) {
     using F = typename Traits<T>::Fixed;
+    if (!compat) return gb_mag<T>(x, shift, 0);
     const T v = dev_ldexp(x, shift);
     const int64_t q = fabs(v) < (T)9.2233720368547758e18 ? (int64_t)v : INT64_MIN;
     return (F)(x < 0 ? (uint64_t)0 - (uint64_t)q : (uint64_t)q);
@@ -851,11 +853,11 @@ template <class T> struct PbPlanes;
 template <> struct PbPlanes<float> {
     uint32_t m[32];
     uint32_t sg;
-    __device__ void load(const float *row, int shift) {
+    __device__ void load(const float *row, int shift, int compat) {
         sg = 0;
 #pragma unroll
         for (int j = 0; j < 32; j++) {
-            m[j] = pb_mag<float>(row[j], shift);
+            m[j] = pb_mag<float>(row[j], shift, compat);
             sg |= (uint32_t)(row[j] < 0) << j;
         }
         transpose32(m);
@@ -865,11 +867,11 @@ template <> struct PbPlanes<float> {
 template <> struct PbPlanes<double> {
     uint32_t lo[64], hi[64];
     uint64_t sg;
-    __device__ void load(const double *row, int shift) {
+    __device__ void load(const double *row, int shift, int compat) {
         sg = 0;
 #pragma unroll
         for (int j = 0; j < 64; j++) {
-            const uint64_t q = pb_mag<double>(row[j], shift);
+            const uint64_t q = pb_mag<double>(row[j], shift, compat);
             lo[j] = (uint32_t)q;
             hi[j] = (uint32_t)(q >> 32);
             sg |= (uint64_t)(row[j] < 0) << j;
@@ -900,7 +902,7 @@ template <class T, class Fn> __device__ __forceinline__ void pb_for_planes(const
 // (valid elements) + (elements whose first 1 is in plane p).
 template <class T>
 __global__ __launch_bounds__(BlkStage<T>::kTPB) void k_pb_count(const T *__restrict__ x, uint32_t n, int32_t exp,
-                                                             const KeyOf<T> *key, int nb,
+                                                             const KeyOf<T> *key, int nb, int compat,
                                                              uint32_t *__restrict__ wg_bits) {
     constexpr int TPB = BlkStage<T>::kTPB, B = BlkStage<T>::kB, PI = BlkStage<T>::kPitch;
     __shared__ T sx[TPB * PI];
@@ -910,7 +912,7 @@ __global__ __launch_bounds__(BlkStage<T>::kTPB) void k_pb_count(const T *__restr
     stage_in<T>(x, n, e0, sx);
     __syncthreads();
     PbPlanes<T> P;
-    P.load(sx + threadIdx.x * PI, nb - level_exp<T>(exp, key));
+    P.load(sx + threadIdx.x * PI, nb - level_exp<T>(exp, key), compat);
     pb_for_planes<T>(P, nb, [&](int p, uint64_t, uint64_t F) {
         const uint32_t s = pb_wave_sum((uint32_t)__popcll(F));
         if (lane_id() == 0 && s) atomicAdd(&cnt[p], s);
@@ -959,7 +961,7 @@ __global__ __launch_bounds__(1024) void k_pb_scan(const uint32_t *__restrict__ w
 // off[wg][p] + min(t B, valid) + (first-1 flags of the lanes below).
 template <class T>
 __global__ __launch_bounds__(BlkStage<T>::kTPB) void k_pb_write(const T *__restrict__ x, uint32_t n, int32_t exp,
-                                                             const KeyOf<T> *key, int nb,
+                                                             const KeyOf<T> *key, int nb, int compat,
                                                              const uint64_t *__restrict__ off,
                                                              uint64_t *__restrict__ streams, uint64_t stride) {
     using R = typename PbRun<T>::R;
@@ -973,7 +975,7 @@ __global__ __launch_bounds__(BlkStage<T>::kTPB) void k_pb_write(const T *__restr
     stage_in<T>(x, n, e0, sx);
     __syncthreads();
     PbPlanes<T> P;
-    P.load(sx + threadIdx.x * PI, nb - level_exp<T>(exp, key));
+    P.load(sx + threadIdx.x * PI, nb - level_exp<T>(exp, key), compat);
     const uint32_t wv = threadIdx.x / 64;
     pb_for_planes<T>(P, nb, [&](int p, uint64_t, uint64_t F) {
         const uint32_t s = pb_wave_sum((uint32_t)__popcll(F));
@@ -1043,26 +1045,26 @@ template size_t pb_workspace_bytes<float>(uint32_t, int);
 template size_t pb_workspace_bytes<double>(uint32_t, int);
 
 template <class T>
-int launch_pb_encode(const T *x, uint32_t n, int32_t exp, const KeyOf<T> *key, int nb, uint64_t *streams,
-                     uint64_t stride, uint32_t *sizes, void *ws, size_t ws_bytes, hipStream_t s) {
+int launch_pb_encode(const T *x, uint32_t n, int32_t exp, const KeyOf<T> *key, int nb, int compat,
+                     uint64_t *streams, uint64_t stride, uint32_t *sizes, void *ws, size_t ws_bytes, hipStream_t s) {
     if (n == 0 || nb < 1 || nb > (int)(8 * sizeof(T))) { set_error("perbit encode: bad n or num_bitplanes"); return kErrArg; }
     if (stride < pb_stream_words(n)) { set_error("perbit encode: stream stride below ceil(2 n / 64) words"); return kErrArg; }
     if (ws_bytes < pb_workspace_bytes<T>(n, nb)) { set_error("perbit encode: workspace too small"); return kErrArg; }
     const uint32_t nwg = pb_nwg<T>(n);
     uint64_t *off = static_cast<uint64_t *>(ws);
     uint32_t *wg_bits = reinterpret_cast<uint32_t *>(off + (size_t)nwg * nb);
-    hipLaunchKernelGGL(k_pb_count<T>, dim3(nwg), dim3(BlkStage<T>::kTPB), 0, s, x, n, exp, key, nb, wg_bits);
+    hipLaunchKernelGGL(k_pb_count<T>, dim3(nwg), dim3(BlkStage<T>::kTPB), 0, s, x, n, exp, key, nb, compat, wg_bits);
     PRODM_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_pb_scan, dim3(nb), dim3(1024), 0, s, wg_bits, nwg, nb, off, streams, stride, sizes);
     PRODM_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_pb_write<T>, dim3(nwg), dim3(BlkStage<T>::kTPB), 0, s, x, n, exp, key, nb, off, streams,
-                       stride);
+    hipLaunchKernelGGL(k_pb_write<T>, dim3(nwg), dim3(BlkStage<T>::kTPB), 0, s, x, n, exp, key, nb, compat, off,
+                       streams, stride);
     PRODM_CHECK_LAUNCH();
     return kOk;
 }
-template int launch_pb_encode<float>(const float *, uint32_t, int32_t, const uint32_t *, int, uint64_t *, uint64_t,
-                                     uint32_t *, void *, size_t, hipStream_t);
-template int launch_pb_encode<double>(const double *, uint32_t, int32_t, const unsigned long long *, int, uint64_t *,
-                                      uint64_t, uint32_t *, void *, size_t, hipStream_t);
+template int launch_pb_encode<float>(const float *, uint32_t, int32_t, const uint32_t *, int, int, uint64_t *,
+                                     uint64_t, uint32_t *, void *, size_t, hipStream_t);
+template int launch_pb_encode<double>(const double *, uint32_t, int32_t, const unsigned long long *, int, int,
+                                      uint64_t *, uint64_t, uint32_t *, void *, size_t, hipStream_t);
 
 }  // namespace prodm

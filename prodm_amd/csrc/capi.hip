@@ -592,14 +592,14 @@ uint64_t prodm_perbit_stream_words(uint32_t n) { return pb_stream_words(n); }
 size_t prodm_perbit_workspace_bytes(uint32_t n, int nb, int dtype) {
     return dtype == PRODM_F64 ? pb_workspace_bytes<double>(n, nb) : pb_workspace_bytes<float>(n, nb);
 }
-int prodm_perbit_encode_f32(const float *d_x, uint32_t n, int32_t exp, int nb, uint64_t *d_streams,
+int prodm_perbit_encode_f32(const float *d_x, uint32_t n, int32_t exp, int nb, int compat, uint64_t *d_streams,
                             uint64_t stride, uint32_t *d_sizes, void *d_ws, size_t ws_bytes, prodm_stream_t stream) {
-    return launch_pb_encode<float>(d_x, n, exp, nullptr, nb, d_streams, stride, d_sizes, d_ws, ws_bytes,
+    return launch_pb_encode<float>(d_x, n, exp, nullptr, nb, compat, d_streams, stride, d_sizes, d_ws, ws_bytes,
                                    (hipStream_t)stream);
 }
-int prodm_perbit_encode_f64(const double *d_x, uint32_t n, int32_t exp, int nb, uint64_t *d_streams,
+int prodm_perbit_encode_f64(const double *d_x, uint32_t n, int32_t exp, int nb, int compat, uint64_t *d_streams,
                             uint64_t stride, uint32_t *d_sizes, void *d_ws, size_t ws_bytes, prodm_stream_t stream) {
-    return launch_pb_encode<double>(d_x, n, exp, nullptr, nb, d_streams, stride, d_sizes, d_ws, ws_bytes,
+    return launch_pb_encode<double>(d_x, n, exp, nullptr, nb, compat, d_streams, stride, d_sizes, d_ws, ws_bytes,
                                     (hipStream_t)stream);
 }
 

@@ -78,8 +78,8 @@ template <class T> size_t gb_decode_workspace_bytes(uint32_t n, int start, int k
 uint64_t pb_stream_words(uint32_t n);
 template <class T> size_t pb_workspace_bytes(uint32_t n, int nb);
 template <class T>
-int launch_pb_encode(const T *x, uint32_t n, int32_t exp, const KeyOf<T> *key, int nb, uint64_t *streams,
-                     uint64_t stride, uint32_t *sizes, void *ws, size_t ws_bytes, hipStream_t s);
+int launch_pb_encode(const T *x, uint32_t n, int32_t exp, const KeyOf<T> *key, int nb, int compat,
+                     uint64_t *streams, uint64_t stride, uint32_t *sizes, void *ws, size_t ws_bytes, hipStream_t s);
 
 // ---- interleave (DirectInterleaver.hpp:12-63) --------------------------------
 template <class T>

@@ -58,7 +58,8 @@ struct BitReader {
                 overrun = true;
                 return 0;
             }
-            buf = w[next++];
+            std::memcpy(&buf, reinterpret_cast<const char *>(w) + 8 * next, 8);  // any alignment
+            next++;
             left = 64;
         }
         const int b = (int)(buf & 1u);

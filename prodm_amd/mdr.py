@@ -208,15 +208,16 @@ class GroupedDecoder:
 
 
 # ---- PerBit ---------------------------------------------------------------------------
-def perbit_encode(x, exp, nb):
+def perbit_encode(x, exp, nb, compat=False):
     """Returns (streams uint64-word tensor (nb, stride), sizes int32 tensor of
-    bytes); stream p is row p[:sizes[p] // 8] (PerBitBPEncoder.hpp:81-143)."""
+    bytes); stream p is row p[:sizes[p] // 8] (PerBitBPEncoder.hpp:81-143).
+    compat reproduces x86's (int64_t) on |v| >= 2^63 or NaN."""
     n = x.numel()
     stride = int(lib().prodm_perbit_stream_words(n))
     streams = torch.empty((nb, max(stride, 1)), dtype=torch.int64, device=x.device)
     sizes = torch.zeros(nb, dtype=torch.int32, device=x.device)
     ws = _ws(lib().prodm_perbit_workspace_bytes(n, nb, _TDT[x.dtype]))
-    check(_fn("prodm_perbit_encode", x.dtype)(_ptr(x), n, int(exp), nb, _ptr(streams), stride, _ptr(sizes), _ptr(ws),
+    check(_fn("prodm_perbit_encode", x.dtype)(_ptr(x), n, int(exp), nb, int(compat), _ptr(streams), stride, _ptr(sizes), _ptr(ws),
                                               ws.numel(), _stream()), "perbit encode")
     return streams, sizes
 

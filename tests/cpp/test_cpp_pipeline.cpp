@@ -9,7 +9,10 @@
 //  3. The squared-error tables written with a squared collector drive an
 //     S-norm reconstruction whose actual relative L2 error is reported.
 //  4. GPU components against each other: encode -> progressive decode
-//     equals one-shot decode; interleave -> reposition is the identity.
+//     equals one-shot decode (negabinary, grouped, per-bit); interleave ->
+//     reposition is the identity.
+//  5. The per-bit encoder with the hierarchical decomposer through the
+//     composers meets each requested L-infinity tolerance.
 //
 // Usage: test_cpp_pipeline <scratch dir>. Exit status 0 on success.
 #include <sys/stat.h>
@@ -213,6 +216,48 @@ static void hierarchical_path(const std::string &root, const std::vector<float> 
     }
 }
 
+// The GE applications' pairing (Synthesizer4GE.hpp:123): hierarchical
+// decomposer with the per-bit encoder, composed from the GPU components;
+// the progressive reconstruction meets each L-infinity tolerance.
+static void perbit_path(const std::string &root, const std::vector<float> &u, const std::vector<uint32_t> &dims,
+                        int L, int nb) {
+    const std::string a = root + "/composed_pb";
+    mkdir(a.c_str(), 0755);
+    Context ctx = make_context();
+    {
+        ComposedRefactor<float, GpuMGARDHierarchicalDecomposer<float>, GpuDirectInterleaver<float>,
+                         GpuPerBitBPEncoder<float, uint32_t>, AdaptiveLevelCompressor, MaxErrorCollector<float>,
+                         ConcatLevelFileWriter>
+            r{GpuMGARDHierarchicalDecomposer<float>(ctx), GpuDirectInterleaver<float>(ctx),
+              GpuPerBitBPEncoder<float, uint32_t>(ctx), AdaptiveLevelCompressor(64), MaxErrorCollector<float>(),
+              ConcatLevelFileWriter(a + "/metadata.bin", level_files(a, L))};
+        r.refactor(u.data(), dims, L, nb);
+    }
+    using Est = MaxErrorEstimatorHB<float>;
+    using Interp = SignExcludeGreedyBasedSizeInterpreter<Est>;
+    ComposedReconstructor<float, GpuMGARDHierarchicalDecomposer<float>, GpuDirectInterleaver<float>,
+                          GpuPerBitBPEncoder<float, uint32_t>, AdaptiveLevelCompressor, Interp, Est,
+                          ConcatLevelFileRetriever>
+        rc{GpuMGARDHierarchicalDecomposer<float>(ctx), GpuDirectInterleaver<float>(ctx),
+           GpuPerBitBPEncoder<float, uint32_t>(ctx), AdaptiveLevelCompressor(64), Interp(Est()),
+           ConcatLevelFileRetriever(a + "/metadata.bin", level_files(a, L))};
+    rc.load_metadata();
+    const double R = range_of(u);
+    // recompose_to_full overwrites the held field (ComposedReconstructor.hpp:123-131),
+    // so a coarse-grid result is only reported
+    int checked = 0;
+    for (double rel : {1e-1, 1e-2, 1e-3, 1e-4, 1e-5}) {
+        float *x = rc.progressive_reconstruct(rel * R);
+        const bool reduced = rc.get_current_dimensions() != dims;
+        const double e = reduced ? -1 : linf(u, x);
+        std::printf("per-bit tol %.1e: composed err %.3e (%s), retrieved %zu bytes\n", rel, e / R,
+                    reduced ? "coarse grid" : "full grid", rc.get_retrieved_size());
+        if (!reduced) EXPECT(e <= rel * R, "per-bit error %.3e above %.3e", e, rel * R);
+        checked += !reduced;
+    }
+    EXPECT(checked >= 3, "per-bit: only %d tolerances reached the full grid", checked);
+}
+
 template <class Encoder>
 static void component_checks(const std::vector<float> &u, int nb, Encoder enc) {
     const int n = (int)u.size() / 7 + 5;
@@ -275,6 +320,8 @@ int main(int argc, char **argv) {
     hierarchical_path(root, u, dims, L, nb);
     component_checks(u, nb, GpuNegaBinaryBPEncoder<float, uint32_t>(ctx));
     component_checks(u, nb, GpuGroupedBPEncoder<float, uint32_t>(ctx));
+    component_checks(u, nb, GpuPerBitBPEncoder<float, uint32_t>(ctx));
+    perbit_path(root, u, dims, L, nb);
     {  // interleave -> reposition is the identity on every level
         GpuDirectInterleaver<float> ilv(ctx);
         const auto ld = level_dims_of(dims, L);

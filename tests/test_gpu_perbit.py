@@ -1,7 +1,9 @@
 """GPU PerBit encode (prodm_perbit_encode_*) against the oracle: the
-streams and their sizes are byte-identical, for f32 / f64, one to many
-workgroups, ragged tails, special values (zeros, -0, NaN, data above
-2^exp), and the host decoder reconstructs the truncated fixed point from
+streams and their sizes are byte-identical (x86-compatible conversion,
+the oracle's), for f32 / f64, one to many workgroups, ragged tails,
+special values (zeros, -0, NaN, data above 2^exp); the default conversion
+keeps the truncated magnitude at |v| >= 2^63 (fp64, 64 planes), checked
+by decoding; the host decoder reconstructs the truncated fixed point from
 the GPU's streams; PerBit level errors are GroupedBP's."""
 import numpy as np
 import pytest
@@ -26,8 +28,8 @@ def level_exp(x):
     return ob.frexp_exp(ob.maxabs(x), x.dtype.type)
 
 
-def encode_gpu(x, e, nb):
-    streams, sizes = mdr.perbit_encode(torch.from_numpy(x).cuda(), e, nb)
+def encode_gpu(x, e, nb, compat=True):
+    streams, sizes = mdr.perbit_encode(torch.from_numpy(x).cuda(), e, nb, compat)
     torch.cuda.synchronize()
     return mdr.perbit_streams(streams, sizes), sizes.cpu().numpy()
 
@@ -66,6 +68,28 @@ def test_perbit_special_values(gpu, dt):
         assert mine == ob.pb_encode(y, 0, nb), nb
 
 
+@pytest.mark.parametrize("dt,nb", [(np.float32, 32), (np.float64, 62), (np.float64, 64)])
+def test_perbit_default_conversion(gpu, dt, nb):
+    """Default conversion: streams equal the oracle's wherever |v| < 2^63;
+    at 64 planes the level's largest elements (|v| in [2^63, 2^64)) keep
+    their truncated magnitude instead of x86's 2^63."""
+    x = field(30000, dt, nb)
+    e = level_exp(x)
+    mine, _ = encode_gpu(x, e, nb, compat=False)
+    ref = ob.pb_encode(x, e, nb)
+    big = np.abs(x.astype(np.float64)) * 2.0 ** (nb - e) >= 2.0 ** 63
+    assert (nb == 64) == bool(big.any())
+    if nb < 64:
+        assert mine == ref
+        return
+    assert mine != ref
+    y = mdr.PerBitDecoder(torch.float64).decode(0, mine, x.size, e, 0, nb)
+    # trunc(|x| 2^(nb-e)) 2^(e-nb), in Python integers (the scaling is exact)
+    t = np.array([float(int(abs(float(v)) * 2.0 ** (nb - e))) for v in x]) * 2.0 ** (e - nb)
+    assert np.array_equal(np.abs(y), t)
+    assert np.array_equal(np.abs(y[big]), np.abs(x[big]))
+
+
 def test_perbit_roundtrip_large(gpu):
     """8 M elements: the host decoder reconstructs trunc(|x| 2^(nb-e)) with
     the signs from the GPU's streams (planes decoded in three steps)."""
@@ -100,7 +124,7 @@ def test_perbit_encode_rejects_bad_arguments(gpu):
     sizes = torch.zeros(33, dtype=torch.int32, device="cuda")
     ws = torch.zeros(1 << 16, dtype=torch.uint8, device="cuda")
     s = mdr._stream()
-    args = lambda n, nb, stride: (mdr._ptr(x), n, 0, nb, mdr._ptr(out), stride, mdr._ptr(sizes), mdr._ptr(ws),
+    args = lambda n, nb, stride: (mdr._ptr(x), n, 0, nb, 0, mdr._ptr(out), stride, mdr._ptr(sizes), mdr._ptr(ws),
                                   ws.numel(), s)
     assert lib.prodm_perbit_encode_f32(*args(0, 8, 8)) != 0        # no elements
     assert lib.prodm_perbit_encode_f32(*args(100, 33, 8)) != 0     # more planes than bits
