@@ -458,6 +458,10 @@ template <class T>
 __device__ inline typename Traits<T>::Fixed gb_mag(T x, int shift, int compat) {
     using F = typename Traits<T>::Fixed;
     const T v = dev_ldexp(x, shift);
+    if constexpr (sizeof(T) == 4) {  // one conversion where the magnitude fits the word
+        const float a = fabsf(v);
+        if (a < 4294967296.0f) return (F)a;
+    }
     if (v != v) return 0;
     const T av = fabs(v);
     if (av < (T)9.2233720368547758e18) {
@@ -809,11 +813,11 @@ template size_t gb_decode_workspace_bytes<double>(uint32_t, int, int);
 // Stream p carries bit nb - 1 - p of every element's magnitude, element by
 // element, and right after an element's first 1 bit (over planes 0..nb-1)
 // its sign; bits fill 64-bit words LSB first (PerBitBPEncoder.hpp:9-39,
-// 96-143). Encode runs in three passes over workgroups of TPB lanes x B
-// consecutive elements: count (first-1 flags per plane), scan (bit offset
-// of every workgroup's run in every stream) and write (each lane expands
-// its plane word with its sign bits; the workgroup assembles the run in
-// LDS and stores whole words).
+// 96-143). Encode runs in three passes over runs of 64 lanes x B
+// consecutive elements (one wave): count (first-1 flags per plane), scan
+// (bit offset of every wave's run in every stream) and write (each lane
+// expands its plane word with its sign bits; the wave assembles its run
+// in LDS and stores whole words).
 
 // T_fp of the reference, sign ? -(int64_t) v : (int64_t) v with
 // v = ldexp(x, nb - exp) (PerBitBPEncoder.hpp:101-104). The conversion of
@@ -825,6 +829,10 @@ __device__ inline typename Traits<T>::Fixed pb_mag(T x, int shift, int compat) {
     using F = typename Traits<T>::Fixed;
     if (!compat) return gb_mag<T>(x, shift, 0);
     const T v = dev_ldexp(x, shift);
+    if constexpr (sizeof(T) == 4) {  // trunc(|v|): the low word of either sign's conversion
+        const float a = fabsf(v);
+        if (a < 4294967296.0f) return (F)a;
+    }
     const int64_t q = fabs(v) < (T)9.2233720368547758e18 ? (int64_t)v : INT64_MIN;
     return (F)(x < 0 ? (uint64_t)0 - (uint64_t)q : (uint64_t)q);
 }
@@ -833,20 +841,9 @@ template <class T> struct PbRun;  // a lane's run: up to 2 B bits
 template <> struct PbRun<float> { using R = uint64_t; };
 template <> struct PbRun<double> { using R = unsigned __int128; };
 
-// sum over the wave of c (0 <= c < 128), by bit slices
-__device__ inline uint32_t pb_wave_sum(uint32_t c) {
-    uint32_t s = 0;
-#pragma unroll
-    for (int b = 0; b < 7; b++) s += (uint32_t)__popcll(ballot64((c >> b) & 1u)) << b;
-    return s;
-}
-// sum of c over the lower lanes of the wave
-__device__ inline uint32_t pb_wave_prefix(uint32_t c) {
-    uint32_t s = 0;
-#pragma unroll
-    for (int b = 0; b < 7; b++) s += lanes_below(ballot64((c >> b) & 1u)) << b;
-    return s;
-}
+// wave-wide exclusive prefix sum and total (DPP reductions of the device library)
+extern "C" __device__ uint32_t __ockl_wfscan_add_u32(uint32_t, bool);
+extern "C" __device__ uint32_t __ockl_wfred_add_u32(uint32_t);
 
 // the lane's magnitudes as plane words: D[k] bit j = bit k of element j
 template <class T> struct PbPlanes;
@@ -863,6 +860,12 @@ template <> struct PbPlanes<float> {
         transpose32(m);
     }
     template <int K> __device__ uint64_t word() const { return m[K]; }
+    // plane words into the lane's own staging row (w[k] = word k)
+    __device__ void store(float *row) const {
+        uint32_t *w = reinterpret_cast<uint32_t *>(row);
+#pragma unroll
+        for (int k = 0; k < 32; k++) w[k] = m[k];
+    }
 };
 template <> struct PbPlanes<double> {
     uint32_t lo[64], hi[64];
@@ -879,121 +882,175 @@ template <> struct PbPlanes<double> {
         transpose64(lo, hi);
     }
     template <int K> __device__ uint64_t word() const { return word64_get<K>(lo, hi); }
+    __device__ void store(double *row) const {
+        uint64_t *w = reinterpret_cast<uint64_t *>(row);
+        static_for<0, 64>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            w[k] = word64_get<k>(lo, hi);
+        });
+    }
 };
 
-// calls f(p, D, F) for the planes p = 0..nb-1 in order (D = the plane word,
-// F = the elements whose first 1 bit is in plane p)
-template <class T, class Fn> __device__ __forceinline__ void pb_for_planes(const PbPlanes<T> &P, int nb, Fn &&f) {
-    constexpr int B = BlkStage<T>::kB;
+// The lane's plane words, moved to its own staging row after the
+// transpose: f(p, D, F) for the planes p = 0..nb-1 in order (D = the plane
+// word, F = the elements whose first 1 bit is in plane p). A loop over LDS
+// words rather than unrolled registers keeps the kernels small.
+template <class T> using PbWord = typename std::conditional<sizeof(T) == 4, uint32_t, uint64_t>::type;
+template <class T, class Fn> __device__ __forceinline__ void pb_for_planes(const PbWord<T> *w, int nb, Fn &&f) {
     uint64_t H = 0;
-    static_for<0, B>([&](auto kc) {
-        constexpr int k = B - 1 - decltype(kc)::value;
-        const int p = nb - 1 - k;
-        if (p >= 0) {
-            const uint64_t D = P.template word<k>();
-            const uint64_t F = D & ~H;
-            H |= D;
-            f(p, D, F);
-        }
-    });
+    for (int p = 0; p < nb; p++) {
+        const uint64_t D = w[nb - 1 - p];  // bit nb - 1 - p of the lane's elements
+        const uint64_t F = D & ~H;
+        H |= D;
+        f(p, D, F);
+    }
 }
 
-// Pass 1: bits per plane of each workgroup's run: wg_bits[wg][p] =
-// (valid elements) + (elements whose first 1 is in plane p).
+// Runs are cut per wave (64 lanes x B consecutive elements): the waves of
+// a workgroup share only the coalesced staging of the input.
+template <class T> static uint32_t pb_nwg(uint32_t n) { return div_up(n, (uint64_t)BlkStage<T>::kTPB * BlkStage<T>::kB); }
+template <class T> static uint32_t pb_nwaves(uint32_t n) { return pb_nwg<T>(n) * (BlkStage<T>::kTPB / 64); }
+
+// valid elements of wave gw
+template <class T> __device__ inline uint32_t pb_wave_valid(uint32_t n, uint32_t gw) {
+    constexpr uint64_t E = 64ull * BlkStage<T>::kB;
+    const uint64_t e = (uint64_t)gw * E;
+    return e < n ? (uint32_t)min(E, (uint64_t)n - e) : 0u;
+}
+
+// Pass 1: bits[p][gw] = bits of wave gw's run in plane p: its valid
+// elements plus those whose first 1 bit is in plane p.
 template <class T>
 __global__ __launch_bounds__(BlkStage<T>::kTPB) void k_pb_count(const T *__restrict__ x, uint32_t n, int32_t exp,
                                                              const KeyOf<T> *key, int nb, int compat,
-                                                             uint32_t *__restrict__ wg_bits) {
-    constexpr int TPB = BlkStage<T>::kTPB, B = BlkStage<T>::kB, PI = BlkStage<T>::kPitch;
+                                                             uint32_t *__restrict__ bits, uint32_t nwv) {
+    constexpr int TPB = BlkStage<T>::kTPB, B = BlkStage<T>::kB, PI = BlkStage<T>::kPitch, NW = TPB / 64;
     __shared__ T sx[TPB * PI];
-    __shared__ uint32_t cnt[64];
-    for (int i = threadIdx.x; i < nb; i += TPB) cnt[i] = 0;
     const uint64_t e0 = (uint64_t)blockIdx.x * TPB * B;
     stage_in<T>(x, n, e0, sx);
     __syncthreads();
-    PbPlanes<T> P;
-    P.load(sx + threadIdx.x * PI, nb - level_exp<T>(exp, key), compat);
-    pb_for_planes<T>(P, nb, [&](int p, uint64_t, uint64_t F) {
-        const uint32_t s = pb_wave_sum((uint32_t)__popcll(F));
-        if (lane_id() == 0 && s) atomicAdd(&cnt[p], s);
+    T *row = sx + threadIdx.x * PI;
+    {
+        PbPlanes<T> P;
+        P.load(row, nb - level_exp<T>(exp, key), compat);
+        P.store(row);
+    }
+    const int lane = (int)lane_id();
+    const uint32_t gw = blockIdx.x * NW + threadIdx.x / 64;
+    uint32_t mine = 0;  // lane p: plane p's first-1 flags of the wave
+    pb_for_planes<T>(reinterpret_cast<const PbWord<T> *>(row), nb, [&](int p, uint64_t, uint64_t F) {
+        const uint32_t s = __ockl_wfred_add_u32((uint32_t)__popcll(F));
+        if (lane == p) mine = s;
     });
+    if (lane < nb) bits[(uint64_t)lane * nwv + gw] = pb_wave_valid<T>(n, gw) + mine;
+}
+
+// Pass 2: exclusive bit offsets off[p][gw] of the wave runs and the stream
+// sizes in bytes (whole words). Blocks of kPbScanBlk waves per plane: their
+// sums, then every block's waves with the sum of the blocks below.
+constexpr int kPbScanT = 1024, kPbScanPer = 4, kPbScanBlk = kPbScanT * kPbScanPer;
+
+// sum over the workgroup (all threads call it); result on thread 0
+__device__ inline uint64_t pb_block_sum(uint64_t v, uint64_t *red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane_id() == 0) red[threadIdx.x / 64] = v;
     __syncthreads();
-    const uint32_t nv = (uint32_t)min((uint64_t)TPB * B, (uint64_t)n - e0);
-    for (int i = threadIdx.x; i < nb; i += TPB) wg_bits[(uint64_t)blockIdx.x * nb + i] = nv + cnt[i];
+    uint64_t t = 0;
+    if (threadIdx.x == 0)
+        for (uint32_t i = 0; i < blockDim.x / 64; i++) t += red[i];
+    return t;
 }
 
-// Pass 2, one workgroup per plane: exclusive bit offsets off[wg][p] of the
-// workgroup runs, the stream size in bytes (whole words), and zeros in
-// the words that two runs share (they are completed by atomic ORs).
-__global__ __launch_bounds__(1024) void k_pb_scan(const uint32_t *__restrict__ wg_bits, uint32_t nwg, int nb,
-                                                  uint64_t *__restrict__ off, uint64_t *__restrict__ streams,
-                                                  uint64_t stride, uint32_t *__restrict__ sizes) {
-    __shared__ uint32_t sh[1024];
-    const int p = blockIdx.x;
-    uint64_t *st = streams + (uint64_t)p * stride;
-    uint64_t carry = 0;
-    for (uint32_t c0 = 0; c0 < nwg; c0 += 1024) {
-        const uint32_t wg = c0 + threadIdx.x;
-        const uint32_t v = wg < nwg ? wg_bits[(uint64_t)wg * nb + p] : 0;
-        sh[threadIdx.x] = v;
-        __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
-            const uint32_t t = threadIdx.x >= (uint32_t)o ? sh[threadIdx.x - o] : 0;
-            __syncthreads();
-            sh[threadIdx.x] += t;
-            __syncthreads();
-        }
-        if (wg < nwg) {
-            const uint64_t o = carry + sh[threadIdx.x] - v;
-            off[(uint64_t)wg * nb + p] = o;
-            if (o & 63) st[o >> 6] = 0;
-        }
-        carry += sh[1023];
-        __syncthreads();
+__global__ __launch_bounds__(kPbScanT) void k_pb_scan_sum(const uint32_t *__restrict__ bits, uint32_t nwv,
+                                                         uint64_t *__restrict__ sums) {
+    __shared__ uint64_t red[kPbScanT / 64];
+    const int p = blockIdx.y;
+    const uint32_t *row = bits + (uint64_t)p * nwv;
+    uint32_t v = 0;
+#pragma unroll
+    for (int i = 0; i < kPbScanPer; i++) {
+        const uint32_t w = blockIdx.x * kPbScanBlk + i * kPbScanT + threadIdx.x;
+        if (w < nwv) v += row[w];
     }
-    if (threadIdx.x == 0) {
-        if (carry & 63) st[carry >> 6] = 0;
-        sizes[p] = (uint32_t)(((carry + 63) >> 6) * 8);
+    const uint64_t t = pb_block_sum(v, red);
+    if (threadIdx.x == 0) sums[(uint64_t)p * gridDim.x + blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kPbScanT) void k_pb_scan_off(const uint32_t *__restrict__ bits, uint32_t nwv,
+                                                         const uint64_t *__restrict__ sums,
+                                                         uint64_t *__restrict__ off, uint32_t *__restrict__ sizes) {
+    __shared__ uint64_t red[kPbScanT / 64];
+    __shared__ uint32_t wtot[kPbScanT / 64];
+    __shared__ uint64_t base_s;
+    const int p = blockIdx.y;
+    const uint32_t blk = blockIdx.x, lane = lane_id(), wv = threadIdx.x / 64;
+    const uint32_t *row = bits + (uint64_t)p * nwv;
+    uint64_t *orow = off + (uint64_t)p * nwv;
+    // bits below this block
+    const uint64_t b = pb_block_sum(threadIdx.x < blk ? sums[(uint64_t)p * gridDim.x + threadIdx.x] : 0, red);
+    if (threadIdx.x == 0) base_s = b;
+    // this thread's waves [w0, w0 + kPbScanPer)
+    const uint32_t w0 = blk * kPbScanBlk + threadIdx.x * kPbScanPer;
+    uint32_t v[kPbScanPer], local = 0;
+#pragma unroll
+    for (int i = 0; i < kPbScanPer; i++) {
+        v[i] = w0 + i < nwv ? row[w0 + i] : 0;
+        local += v[i];
+    }
+    const uint32_t pre = __ockl_wfscan_add_u32(local, false);
+    if (lane == 63) wtot[wv] = pre + local;
+    __syncthreads();
+    uint64_t o = base_s + pre;
+    for (uint32_t i = 0; i < wv; i++) o += wtot[i];
+#pragma unroll
+    for (int i = 0; i < kPbScanPer; i++) {
+        if (w0 + i < nwv) {
+            orow[w0 + i] = o;
+            o += v[i];
+            if (w0 + i == nwv - 1) sizes[p] = (uint32_t)(((o + 63) >> 6) * 8);  // the stream's end
+        }
     }
 }
 
-// Pass 3: the streams. Lane t's run in plane p starts at bit
-// off[wg][p] + min(t B, valid) + (first-1 flags of the lanes below).
+// Pass 3: the streams. Lane t of wave gw starts its run in plane p at bit
+// off[p][gw] + min(t B, valid) + (first-1 flags of the lanes below). The
+// wave assembles its run in its own LDS words (in wave order, no
+// barriers) and stores them.
 template <class T>
 __global__ __launch_bounds__(BlkStage<T>::kTPB) void k_pb_write(const T *__restrict__ x, uint32_t n, int32_t exp,
                                                              const KeyOf<T> *key, int nb, int compat,
-                                                             const uint64_t *__restrict__ off,
-                                                             uint64_t *__restrict__ streams, uint64_t stride) {
+                                                             const uint64_t *__restrict__ off, uint32_t nwv,
+                                                             uint64_t *__restrict__ streams, uint64_t stride,
+                                                             uint64_t *__restrict__ side) {
     using R = typename PbRun<T>::R;
     constexpr int TPB = BlkStage<T>::kTPB, B = BlkStage<T>::kB, PI = BlkStage<T>::kPitch, NW = TPB / 64;
-    constexpr int kRunWords = (2 * TPB * B + 63) / 64 + 2;  // a workgroup run plus its start offset
+    constexpr int kRunWords = (2 * 64 * B + 63) / 64 + 2;  // a wave's run plus its start offset
     __shared__ T sx[TPB * PI];
-    __shared__ unsigned long long run[2][kRunWords];
-    __shared__ uint32_t wt[64][NW];  // first-1 flags per plane and wave
-    for (int i = threadIdx.x; i < 2 * kRunWords; i += TPB) (&run[0][0])[i] = 0;
+    __shared__ unsigned long long run[NW][kRunWords];
+    const int lane = (int)lane_id();
+    const uint32_t wv = threadIdx.x / 64, gw = blockIdx.x * NW + wv;
+    unsigned long long *buf = run[wv];
+    for (int i = lane; i < kRunWords; i += 64) buf[i] = 0;
+    const uint64_t my_off = lane < nb ? off[(uint64_t)lane * nwv + gw] : 0;  // lane p: plane p's offset
     const uint64_t e0 = (uint64_t)blockIdx.x * TPB * B;
     stage_in<T>(x, n, e0, sx);
     __syncthreads();
-    PbPlanes<T> P;
-    P.load(sx + threadIdx.x * PI, nb - level_exp<T>(exp, key), compat);
-    const uint32_t wv = threadIdx.x / 64;
-    pb_for_planes<T>(P, nb, [&](int p, uint64_t, uint64_t F) {
-        const uint32_t s = pb_wave_sum((uint32_t)__popcll(F));
-        if (lane_id() == 0) wt[p][wv] = s;
-    });
-    __syncthreads();
-    const uint32_t nv = (uint32_t)min((uint64_t)TPB * B, (uint64_t)n - e0);
-    const uint32_t below = min((uint32_t)threadIdx.x * B, nv);  // valid elements of the lanes below
-    const uint64_t S = P.sg;
-    pb_for_planes<T>(P, nb, [&](int p, uint64_t D, uint64_t F) {
-        unsigned long long *buf = run[p & 1];
-        const uint64_t o = off[(uint64_t)blockIdx.x * nb + p];
-        uint32_t flags = 0, wbelow = 0;
-        for (int w = 0; w < NW; w++) {
-            flags += wt[p][w];
-            if (w < (int)wv) wbelow += wt[p][w];
-        }
-        const uint32_t q = (uint32_t)(o & 63) + below + wbelow + pb_wave_prefix((uint32_t)__popcll(F));
+    T *row = sx + threadIdx.x * PI;
+    uint64_t S;  // sign bits of the lane's elements
+    {
+        PbPlanes<T> P;
+        P.load(row, nb - level_exp<T>(exp, key), compat);
+        P.store(row);
+        S = P.sg;
+    }
+    const uint32_t nv = pb_wave_valid<T>(n, gw);
+    const uint32_t below = min((uint32_t)lane * B, nv);  // valid elements of the lanes below
+    pb_for_planes<T>(reinterpret_cast<const PbWord<T> *>(row), nb, [&](int p, uint64_t D, uint64_t F) {
+        const uint64_t o = ((uint64_t)readlane_u32((uint32_t)(my_off >> 32), p) << 32) |
+                           readlane_u32((uint32_t)my_off, p);
+        const uint32_t c = (uint32_t)__popcll(F);
+        const uint32_t flags = __ockl_wfred_add_u32(c);
+        const uint32_t q = (uint32_t)(o & 63) + below + __ockl_wfscan_add_u32(c, false);
         // insert each flagged element's sign after its bit, highest first
         R r = (R)D;
         uint64_t ff = F;
@@ -1017,29 +1074,47 @@ __global__ __launch_bounds__(BlkStage<T>::kTPB) void k_pb_write(const T *__restr
             if (w1) atomicOr(&buf[wi + 1], w1);
             if (w2) atomicOr(&buf[wi + 2], w2);
         }
-        __syncthreads();
-        // whole words of this run are stored; the first and last may be
-        // shared with the neighbouring runs (zeroed by the scan) and are ORed
+        // whole words of this run are stored; a first word shared with the
+        // run below (head) and a last word not wholly this run's (tail) go to
+        // side[p][gw] for k_pb_fixup
         const uint64_t end = o + nv + flags;
         const uint32_t nwords = (uint32_t)(((o & 63) + nv + flags + 63) >> 6);
         uint64_t *st = streams + (uint64_t)p * stride + (o >> 6);
-        for (uint32_t w = threadIdx.x; w < nwords; w += TPB) {
+        uint64_t *sd = side + 2 * ((uint64_t)p * nwv + gw);
+        for (uint32_t w = lane; w < nwords; w += 64) {
             const uint64_t v = buf[w];
             buf[w] = 0;
-            const uint64_t gb = ((o >> 6) + w) << 6;  // first bit of the word
-            if (gb >= o && gb + 64 <= end) st[w] = v;
-            else if (v) atomicOr((unsigned long long *)&st[w], (unsigned long long)v);
+            const bool head = w == 0 && (o & 63);
+            if (head) sd[0] = v;
+            else if (w == nwords - 1 && (end & 63)) sd[1] = v;
+            else st[w] = v;
         }
     });
 }
 
-template <class T> static uint32_t pb_nwg(uint32_t n) { return div_up(n, (uint64_t)BlkStage<T>::kTPB * BlkStage<T>::kB); }
+// Pass 4: the words at run boundaries. Only the last non-empty run can be
+// shorter than a word (the others hold 64 B elements), so a word is shared
+// by at most two runs: run g's head word is its head ORed with run g - 1's
+// tail; the last run's tail word is its own.
+__global__ __launch_bounds__(256) void k_pb_fixup(const uint32_t *__restrict__ bits, const uint64_t *__restrict__ off,
+                                                  const uint64_t *__restrict__ side, uint32_t nwv, uint32_t glast,
+                                                  uint64_t *__restrict__ streams, uint64_t stride) {
+    const uint32_t g = blockIdx.x * 256 + threadIdx.x;
+    const int p = blockIdx.y;
+    if (g > glast) return;
+    const uint64_t i = (uint64_t)p * nwv + g;
+    const uint64_t o = off[i], end = o + bits[i];
+    uint64_t *st = streams + (uint64_t)p * stride;
+    if (o & 63) st[o >> 6] = side[2 * i] | side[2 * (i - 1) + 1];
+    const bool one_word = (o & 63) && ((o & 63) + bits[i] <= 64);  // the head word is also the last
+    if (g == glast && (end & 63) && !one_word) st[(end - 1) >> 6] = side[2 * i + 1];
+}
 
 uint64_t pb_stream_words(uint32_t n) { return (2ull * n + 63) / 64; }
 
 template <class T> size_t pb_workspace_bytes(uint32_t n, int nb) {
-    const uint64_t nwg = pb_nwg<T>(n);
-    return nwg * nb * 8 + nwg * nb * 4 + 256;
+    const uint64_t nwv = pb_nwaves<T>(n), nblk = div_up(nwv, kPbScanBlk);
+    return nwv * nb * (8 + 16 + 4) + nblk * nb * 8 + 256;
 }
 template size_t pb_workspace_bytes<float>(uint32_t, int);
 template size_t pb_workspace_bytes<double>(uint32_t, int);
@@ -1050,14 +1125,24 @@ int launch_pb_encode(const T *x, uint32_t n, int32_t exp, const KeyOf<T> *key, i
     if (n == 0 || nb < 1 || nb > (int)(8 * sizeof(T))) { set_error("perbit encode: bad n or num_bitplanes"); return kErrArg; }
     if (stride < pb_stream_words(n)) { set_error("perbit encode: stream stride below ceil(2 n / 64) words"); return kErrArg; }
     if (ws_bytes < pb_workspace_bytes<T>(n, nb)) { set_error("perbit encode: workspace too small"); return kErrArg; }
-    const uint32_t nwg = pb_nwg<T>(n);
+    const uint32_t nwg = pb_nwg<T>(n), nwv = pb_nwaves<T>(n);
+    const uint32_t nblk = div_up(nwv, kPbScanBlk);
+    if (nblk > kPbScanT) { set_error("perbit encode: too many elements"); return kErrArg; }
     uint64_t *off = static_cast<uint64_t *>(ws);
-    uint32_t *wg_bits = reinterpret_cast<uint32_t *>(off + (size_t)nwg * nb);
-    hipLaunchKernelGGL(k_pb_count<T>, dim3(nwg), dim3(BlkStage<T>::kTPB), 0, s, x, n, exp, key, nb, compat, wg_bits);
+    uint64_t *side = off + (size_t)nwv * nb;
+    uint64_t *sums = side + 2 * (size_t)nwv * nb;
+    uint32_t *bits = reinterpret_cast<uint32_t *>(sums + (size_t)nblk * nb);
+    hipLaunchKernelGGL(k_pb_count<T>, dim3(nwg), dim3(BlkStage<T>::kTPB), 0, s, x, n, exp, key, nb, compat, bits, nwv);
     PRODM_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_pb_scan, dim3(nb), dim3(1024), 0, s, wg_bits, nwg, nb, off, streams, stride, sizes);
+    hipLaunchKernelGGL(k_pb_scan_sum, dim3(nblk, nb), dim3(kPbScanT), 0, s, bits, nwv, sums);
     PRODM_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_pb_write<T>, dim3(nwg), dim3(BlkStage<T>::kTPB), 0, s, x, n, exp, key, nb, compat, off,
+    hipLaunchKernelGGL(k_pb_scan_off, dim3(nblk, nb), dim3(kPbScanT), 0, s, bits, nwv, sums, off, sizes);
+    PRODM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_pb_write<T>, dim3(nwg), dim3(BlkStage<T>::kTPB), 0, s, x, n, exp, key, nb, compat, off, nwv,
+                       streams, stride, side);
+    PRODM_CHECK_LAUNCH();
+    const uint32_t glast = (n - 1) / (64u * BlkStage<T>::kB);  // the last wave with elements
+    hipLaunchKernelGGL(k_pb_fixup, dim3(div_up(glast + 1, 256), nb), dim3(256), 0, s, bits, off, side, nwv, glast,
                        streams, stride);
     PRODM_CHECK_LAUNCH();
     return kOk;
