@@ -185,6 +185,52 @@ def batched_report(steps, warmup, nb=32):
     return res
 
 
+def perbit_report(steps, warmup, n=513, L=6, nb=32):
+    """PerBit encode (PerBitBPEncoder.hpp:81-143) of the finest level of the
+    n^3 workload under the hierarchical decomposer (the pairing of ProDM's GE
+    applications): the level's coefficients are resident in HBM; a step is one
+    encode of all nb streams (count, scan, write and boundary passes), timed
+    with HIP events on the launch stream. Bytes = coefficients read + stream
+    bytes written."""
+    import torch
+    from prodm_amd import capi, mdr
+    u = make_field_gpu(n, seed=0)
+    plan = mdr.Plan([n, n, n], L, nb, torch.float32, capi.NEGABINARY, capi.HIERARCHICAL)
+    plan.refactor(u)
+    x = plan.coefficients(L).contiguous()
+    _, e, _ = plan.level_info(L)  # the level's exponent (frexp of its max)
+    plan.close()
+    del u
+    m = x.numel()
+    stride = int(capi.lib().prodm_perbit_stream_words(m))
+    streams = torch.empty((nb, stride), dtype=torch.int64, device="cuda")
+    sizes = torch.zeros(nb, dtype=torch.int32, device="cuda")
+    ws = torch.empty(int(capi.lib().prodm_perbit_workspace_bytes(m, nb, capi.F32)), dtype=torch.uint8, device="cuda")
+    s = mdr._stream()
+
+    def enc():
+        capi.check(capi.lib().prodm_perbit_encode_f32(mdr._ptr(x), m, e, nb, 0, mdr._ptr(streams), stride,
+                                                      mdr._ptr(sizes), mdr._ptr(ws), ws.numel(), s), "perbit encode")
+    for _ in range(max(warmup, 1)):
+        enc()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        enc()
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    out_bytes = int(sizes.to(torch.int64).sum().item())
+    total = 4 * m + out_bytes
+    gbs = total / (ms * 1e6)
+    log(f"[bench] perbit: {m} coefficients, {ms:.3f} ms per encode, {gbs:.1f} GB/s (in + out)")
+    return {"workload": f"PerBit encode of the finest level ({m} float32 coefficients) of the {n}^3 field, "
+                        f"hierarchical decomposer, {L} levels, {nb} planes",
+            "elements": m, "ms": round(ms, 4), "in_bytes": 4 * m, "stream_bytes": out_bytes,
+            "GBps": round(gbs, 1), "hbm_frac": round(gbs / PEAK_HBM_GBS, 4)}
+
+
 def plan_planes(level_bounds, level_plane_bytes, nb, tol):
     """Planes per level for an L-infinity tolerance: the host C++ greedy
     planner (GreedyBasedSizeInterpreter.hpp:23-70) over the MaxErrorCollector
@@ -275,6 +321,8 @@ def main():
     ap.add_argument("--graph", action="store_true", help="time HIP graph replays instead of eager launches")
     ap.add_argument("--no-batched", action="store_true", help="skip the 96-field batched refactor report")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched report (one JSON line)")
+    ap.add_argument("--no-perbit", action="store_true", help="skip the PerBit encode report")
+    ap.add_argument("--perbit-only", action="store_true", help="run only the PerBit encode report (one JSON line)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only a plain copy kernel of known traffic (for counter scaling) and exit")
     args = ap.parse_args()
@@ -291,6 +339,9 @@ def main():
 
     if args.batched_only:
         print(json.dumps(batched_report(args.steps, args.warmup)), flush=True)
+        return
+    if args.perbit_only:
+        print(json.dumps(perbit_report(args.steps, args.warmup)), flush=True)
         return
 
     n, L, nb = args.size, args.levels, args.bitplanes
@@ -484,6 +535,8 @@ def main():
     plan.close()
     if not args.no_batched:
         res["batched"] = batched_report(args.steps, args.warmup)
+    if not args.no_perbit:
+        res["perbit"] = perbit_report(args.steps, args.warmup)
     print(json.dumps(res), flush=True)
 
 

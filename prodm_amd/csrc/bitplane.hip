@@ -891,6 +891,48 @@ template <> struct PbPlanes<double> {
     }
 };
 
+// f32: the lane's 32 elements straight from global memory (8 x 16 B when
+// the row is whole and aligned) and its plane words kept in registers, so
+// the workgroup needs no staging LDS.
+struct PbLane32 {
+    uint32_t m[32];
+    uint32_t sg;
+    __device__ void load(const float *__restrict__ x, uint64_t e, uint32_t n, int shift, int compat, bool vec) {
+        float v[32];
+        if (vec && e + 32 <= n) {
+            const float4 *q = reinterpret_cast<const float4 *>(x + e);
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const float4 t = q[i];
+                v[4 * i] = t.x; v[4 * i + 1] = t.y; v[4 * i + 2] = t.z; v[4 * i + 3] = t.w;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 32; j++) v[j] = e + j < n ? x[e + j] : 0.f;
+        }
+        sg = 0;
+#pragma unroll
+        for (int j = 0; j < 32; j++) {
+            m[j] = pb_mag<float>(v[j], shift, compat);
+            sg |= (uint32_t)(v[j] < 0) << j;
+        }
+        transpose32(m);
+    }
+};
+template <class Fn> __device__ __forceinline__ void pb_for_planes(const PbLane32 &P, int nb, Fn &&f) {
+    uint64_t H = 0;
+    static_for<0, 32>([&](auto kc) {  // unrolled: the words stay in registers
+        constexpr int k = 31 - decltype(kc)::value;
+        const int p = nb - 1 - k;
+        if (p >= 0) {
+            const uint64_t D = P.m[k];
+            const uint64_t F = D & ~H;
+            H |= D;
+            f(p, D, F);
+        }
+    });
+}
+
 // The lane's plane words, moved to its own staging row after the
 // transpose: f(p, D, F) for the planes p = 0..nb-1 in order (D = the plane
 // word, F = the elements whose first 1 bit is in plane p). A loop over LDS
@@ -925,23 +967,30 @@ __global__ __launch_bounds__(BlkStage<T>::kTPB) void k_pb_count(const T *__restr
                                                              const KeyOf<T> *key, int nb, int compat,
                                                              uint32_t *__restrict__ bits, uint32_t nwv) {
     constexpr int TPB = BlkStage<T>::kTPB, B = BlkStage<T>::kB, PI = BlkStage<T>::kPitch, NW = TPB / 64;
-    __shared__ T sx[TPB * PI];
-    const uint64_t e0 = (uint64_t)blockIdx.x * TPB * B;
-    stage_in<T>(x, n, e0, sx);
-    __syncthreads();
-    T *row = sx + threadIdx.x * PI;
-    {
-        PbPlanes<T> P;
-        P.load(row, nb - level_exp<T>(exp, key), compat);
-        P.store(row);
-    }
     const int lane = (int)lane_id();
     const uint32_t gw = blockIdx.x * NW + threadIdx.x / 64;
+    const int shift = nb - level_exp<T>(exp, key);
     uint32_t mine = 0;  // lane p: plane p's first-1 flags of the wave
-    pb_for_planes<T>(reinterpret_cast<const PbWord<T> *>(row), nb, [&](int p, uint64_t, uint64_t F) {
+    auto count = [&](int p, uint64_t, uint64_t F) {
         const uint32_t s = __ockl_wfred_add_u32((uint32_t)__popcll(F));
         if (lane == p) mine = s;
-    });
+    };
+    if constexpr (sizeof(T) == 4) {
+        PbLane32 P;
+        P.load(x, ((uint64_t)blockIdx.x * TPB + threadIdx.x) * B, n, shift, compat, ((uintptr_t)x & 15) == 0);
+        pb_for_planes(P, nb, count);
+    } else {
+        __shared__ T sx[TPB * PI];
+        stage_in<T>(x, n, (uint64_t)blockIdx.x * TPB * B, sx);
+        __syncthreads();
+        T *row = sx + threadIdx.x * PI;
+        {
+            PbPlanes<T> P;
+            P.load(row, shift, compat);
+            P.store(row);
+        }
+        pb_for_planes<T>(reinterpret_cast<const PbWord<T> *>(row), nb, count);
+    }
     if (lane < nb) bits[(uint64_t)lane * nwv + gw] = pb_wave_valid<T>(n, gw) + mine;
 }
 
@@ -1025,27 +1074,17 @@ __global__ __launch_bounds__(BlkStage<T>::kTPB) void k_pb_write(const T *__restr
     using R = typename PbRun<T>::R;
     constexpr int TPB = BlkStage<T>::kTPB, B = BlkStage<T>::kB, PI = BlkStage<T>::kPitch, NW = TPB / 64;
     constexpr int kRunWords = (2 * 64 * B + 63) / 64 + 2;  // a wave's run plus its start offset
-    __shared__ T sx[TPB * PI];
     __shared__ unsigned long long run[NW][kRunWords];
     const int lane = (int)lane_id();
     const uint32_t wv = threadIdx.x / 64, gw = blockIdx.x * NW + wv;
     unsigned long long *buf = run[wv];
     for (int i = lane; i < kRunWords; i += 64) buf[i] = 0;
     const uint64_t my_off = lane < nb ? off[(uint64_t)lane * nwv + gw] : 0;  // lane p: plane p's offset
-    const uint64_t e0 = (uint64_t)blockIdx.x * TPB * B;
-    stage_in<T>(x, n, e0, sx);
-    __syncthreads();
-    T *row = sx + threadIdx.x * PI;
-    uint64_t S;  // sign bits of the lane's elements
-    {
-        PbPlanes<T> P;
-        P.load(row, nb - level_exp<T>(exp, key), compat);
-        P.store(row);
-        S = P.sg;
-    }
+    const int shift = nb - level_exp<T>(exp, key);
     const uint32_t nv = pb_wave_valid<T>(n, gw);
     const uint32_t below = min((uint32_t)lane * B, nv);  // valid elements of the lanes below
-    pb_for_planes<T>(reinterpret_cast<const PbWord<T> *>(row), nb, [&](int p, uint64_t D, uint64_t F) {
+    uint64_t S;  // sign bits of the lane's elements
+    auto write = [&](int p, uint64_t D, uint64_t F) {
         const uint64_t o = ((uint64_t)readlane_u32((uint32_t)(my_off >> 32), p) << 32) |
                            readlane_u32((uint32_t)my_off, p);
         const uint32_t c = (uint32_t)__popcll(F);
@@ -1089,7 +1128,25 @@ __global__ __launch_bounds__(BlkStage<T>::kTPB) void k_pb_write(const T *__restr
             else if (w == nwords - 1 && (end & 63)) sd[1] = v;
             else st[w] = v;
         }
-    });
+    };
+    if constexpr (sizeof(T) == 4) {
+        PbLane32 P;
+        P.load(x, ((uint64_t)blockIdx.x * TPB + threadIdx.x) * B, n, shift, compat, ((uintptr_t)x & 15) == 0);
+        S = P.sg;
+        pb_for_planes(P, nb, write);
+    } else {
+        __shared__ T sx[TPB * PI];
+        stage_in<T>(x, n, (uint64_t)blockIdx.x * TPB * B, sx);
+        __syncthreads();
+        T *row = sx + threadIdx.x * PI;
+        {
+            PbPlanes<T> P;
+            P.load(row, shift, compat);
+            P.store(row);
+            S = P.sg;
+        }
+        pb_for_planes<T>(reinterpret_cast<const PbWord<T> *>(row), nb, write);
+    }
 }
 
 // Pass 4: the words at run boundaries. Only the last non-empty run can be
