@@ -231,6 +231,50 @@ def perbit_report(steps, warmup, n=513, L=6, nb=32):
             "GBps": round(gbs, 1), "hbm_frac": round(gbs / PEAK_HBM_GBS, 4)}
 
 
+def end_to_end_report(n=513, L=6, nb=32, reps=3, tols=(1e-2, 1e-4, 1e-6)):
+    """Host buffers and files end to end, with the C++ applications over the
+    library: prodm_refactor (upload, GPU stages, plane streams copied out
+    overlapped with host ZSTD, ProDM's files) and prodm_reconstruct (plan,
+    read, decompress, upload, decode + recompose, download, error check) on
+    the n^3 workload written as a raw file in a temporary directory. Reports
+    the best of `reps` refactors (each app run is its own process)."""
+    import re
+    import subprocess
+    import tempfile
+    here = os.path.dirname(os.path.abspath(__file__))
+    binr = os.path.join(here, "prodm_amd", "bin")
+    u = make_field_gpu(n, seed=0).cpu().numpy()
+    out = {"workload": f"{n}^3 float32 raw file -> ProDM files ({L} levels, {nb} planes, NegaBinary, "
+                       f"AdaptiveLevelCompressor(64)) -> progressive retrieval", "refactor": [], "reconstruct": []}
+    with tempfile.TemporaryDirectory() as tmp:
+        raw, rd = os.path.join(tmp, "field.raw"), os.path.join(tmp, "rd")
+        u.tofile(raw)
+        for _ in range(reps):
+            r = subprocess.run([os.path.join(binr, "prodm_refactor"), raw, str(L), str(nb), "3", str(n), str(n), str(n),
+                                "--out", rd], capture_output=True, text=True, timeout=300)
+            if r.returncode:
+                raise RuntimeError(r.stdout + r.stderr)
+            st = [float(v) for v in re.findall(r"([0-9.]+)s", r.stdout)]
+            out["refactor"].append({"device_s": st[0], "copyout_lossless_s": st[1], "files_s": st[2], "total_s": st[3]})
+        files = sum(os.path.getsize(os.path.join(rd, f)) for f in os.listdir(rd))
+        r = subprocess.run([os.path.join(binr, "prodm_reconstruct"), raw, str(len(tols)), *[f"{t:g}" for t in tols],
+                            "--in", rd], capture_output=True, text=True, timeout=300)
+        if r.returncode:
+            raise RuntimeError(r.stdout + r.stderr)
+        for line in r.stdout.splitlines():
+            m = re.search(r"tolerance ([0-9.e+-]+): reconstruct ([0-9.]+)s, retrieved ([0-9]+) bytes.*rel ([0-9.e+-]+)\)",
+                          line)
+            if m:
+                out["reconstruct"].append({"tolerance": float(m.group(1)), "s": float(m.group(2)),
+                                           "file_bytes": int(m.group(3)), "rel_linf": float(m.group(4))})
+    best = min(out["refactor"], key=lambda d: d["total_s"])
+    out["file_bytes"] = files
+    out["refactor_GBps"] = round(u.nbytes / best["total_s"] / 1e9, 3)
+    log(f"[bench] end to end: refactor {best['total_s']:.3f} s ({out['refactor_GBps']} GB/s of input), "
+        f"reconstruct {[d['s'] for d in out['reconstruct']]} s")
+    return out
+
+
 def plan_planes(level_bounds, level_plane_bytes, nb, tol):
     """Planes per level for an L-infinity tolerance: the host C++ greedy
     planner (GreedyBasedSizeInterpreter.hpp:23-70) over the MaxErrorCollector
@@ -323,6 +367,8 @@ def main():
     ap.add_argument("--batched-only", action="store_true", help="run only the batched report (one JSON line)")
     ap.add_argument("--no-perbit", action="store_true", help="skip the PerBit encode report")
     ap.add_argument("--perbit-only", action="store_true", help="run only the PerBit encode report (one JSON line)")
+    ap.add_argument("--end-to-end", action="store_true",
+                    help="run only the host-buffer + files report of the C++ applications (one JSON line)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only a plain copy kernel of known traffic (for counter scaling) and exit")
     args = ap.parse_args()
@@ -342,6 +388,9 @@ def main():
         return
     if args.perbit_only:
         print(json.dumps(perbit_report(args.steps, args.warmup)), flush=True)
+        return
+    if args.end_to_end:
+        print(json.dumps(end_to_end_report()), flush=True)
         return
 
     n, L, nb = args.size, args.levels, args.bitplanes

@@ -4,7 +4,8 @@
 // GpuRefactor runs the GPU stages of ComposedRefactor::refactor
 // (Refactor/ComposedRefactor.hpp:78-141: decompose, interleave, max-abs,
 // encode) without leaving the device, then copies the plane streams out once
-// and hands them to the host lossless stage and the writer. GpuReconstructor
+// (overlapped with the host lossless stage, level by level) and hands them
+// to the compressor and the writer. GpuReconstructor
 // reads ProDM's files, plans with a size interpreter, decompresses on the
 // host, and decodes and recomposes on the device
 // (Reconstructor/ComposedReconstructor.hpp:27-258). Both use the same files
@@ -18,6 +19,7 @@
 #ifndef PRODM_GPU_MDR_HPP
 #define PRODM_GPU_MDR_HPP
 
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -27,6 +29,7 @@
 #include <vector>
 
 #include "composed.hpp"
+#include "compressors.hpp"  // parallel_for
 #include "device.hpp"
 #include "io.hpp"
 
@@ -60,22 +63,39 @@ class GpuRefactor {
 
     // Host input (copied to the device) ...
     void refactor(T const *data, const std::vector<uint32_t> &dims, uint8_t target_level, uint8_t num_bitplanes) {
+        const auto t0 = std::chrono::steady_clock::now();
         prepare(dims, target_level, num_bitplanes);
         T *d = in_.as<T>(product(dims));
         to_device(d, data, product(dims), stream_);
+        hip_check(hipStreamSynchronize(stream_), "upload");
+        const double up = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         refactor_device(d, dims, target_level, num_bitplanes);
+        times_.device += up;
     }
     // ... or a device array (dense, dims[0] slowest).
+    //
+    // The plane streams of all levels are copied to pinned host memory in one
+    // pass of asynchronous copies (a strided copy per level), coarsest level
+    // first, each level followed by an event; level l is compressed on host
+    // threads as soon as its event completes, while the copies of the finer
+    // levels continue.
     void refactor_device(const T *d_data, const std::vector<uint32_t> &dims, uint8_t target_level,
                          uint8_t num_bitplanes) {
+        using clock = std::chrono::steady_clock;
+        const auto t0 = clock::now();
         prepare(dims, target_level, num_bitplanes);
         check(prodm_mdr_refactor(plan_->get(), d_data, stream_), "refactor");
-        const int L = target_level;
+        const int L = target_level, nb = num_bitplanes;
+        const bool grouped = encoder_ == PRODM_ENC_GROUPED;
         MdrMetadata<T> m;
         m.dims = dims;
-        m.negabinary = encoder_ == PRODM_ENC_NEGABINARY;
+        m.negabinary = !grouped;
         std::vector<std::vector<uint8_t *>> comps(L + 1);
         m.level_sizes.assign(L + 1, {});
+        // plane layout (level info synchronizes: the device stages are done)
+        std::vector<std::vector<const uint8_t *>> src(L + 1);
+        std::vector<uint64_t> head(L + 1, 0), off(L + 2, 0);
+        std::vector<const uint8_t *> rb(L + 1, nullptr);
         for (int l = 0; l <= L; l++) {
             double mx;
             int32_t e;
@@ -83,26 +103,112 @@ class GpuRefactor {
             check(prodm_mdr_level_info(plan_->get(), l, &mx, &e, &n), "level info");
             const T tmx = (T)mx;
             m.level_error_bounds.push_back(m.negabinary ? tmx * 4 : tmx);
-            if (squared_) m.level_squared_errors.push_back(collect_squared(l, n, num_bitplanes, tmx));
-            for (int p = 0; p < num_bitplanes; p++) {
-                uint64_t bytes = 0;
-                check(prodm_mdr_plane_stream(plan_->get(), l, p, nullptr, 0, &bytes, stream_), "plane size");
-                uint8_t *buf = static_cast<uint8_t *>(std::malloc(bytes ? bytes : 1));
-                check(prodm_mdr_plane_stream(plan_->get(), l, p, buf, bytes, &bytes, stream_), "plane stream");
-                comps[l].push_back(buf);
-                m.level_sizes[l].push_back((uint32_t)bytes);
+            if (squared_) m.level_squared_errors.push_back(collect_squared(l, n, nb, tmx));
+            if (grouped) {  // stream 0 carries u32 nblocks + u8 rb[nblocks] (GroupedBPEncoder.hpp:64-71)
+                uint64_t nblk = 0;
+                check(prodm_mdr_level_rb(plan_->get(), l, &rb[l], &nblk), "level rb");
+                head[l] = 4 + nblk;
             }
+            uint64_t sum = head[l];
+            for (int p = 0; p < nb; p++) {
+                const void *d = nullptr;
+                uint64_t bytes = 0;
+                check(prodm_mdr_plane(plan_->get(), l, p, &d, &bytes), "plane");
+                src[l].push_back(static_cast<const uint8_t *>(d));
+                m.level_sizes[l].push_back((uint32_t)(bytes + (p == 0 ? head[l] : 0)));
+                sum += bytes;
+            }
+            off[l + 1] = off[l] + ((sum + 255) & ~255ull);
+        }
+        const auto t1 = clock::now();
+        uint8_t *h = pinned_->get(off[L + 1]);
+        events_->ensure(L + 1);
+        for (int l = 0; l <= L; l++) {
+            uint8_t *dst = h + off[l];
+            if (grouped) {
+                const uint32_t nblk = (uint32_t)(head[l] - 4);
+                std::memcpy(dst, &nblk, 4);
+                hip_check(hipMemcpyAsync(dst + 4, rb[l], nblk, hipMemcpyDeviceToHost, stream_), "rb copy");
+                dst += head[l];
+                for (int p = 0; p < nb; p++) {
+                    const uint64_t bytes = m.level_sizes[l][p] - (p == 0 ? head[l] : 0);
+                    if (bytes) hip_check(hipMemcpyAsync(dst, src[l][p], bytes, hipMemcpyDeviceToHost, stream_), "plane copy");
+                    dst += bytes;
+                }
+            } else if (m.level_sizes[l][0]) {  // equal planes at a fixed stride: one strided copy
+                const uint64_t w = m.level_sizes[l][0];
+                const uint64_t pitch = nb > 1 ? (uint64_t)(src[l][1] - src[l][0]) : w;
+                hip_check(hipMemcpy2DAsync(dst, w, src[l][0], pitch, w, nb, hipMemcpyDeviceToHost, stream_),
+                          "plane copy");
+            }
+            hip_check(hipEventRecord(events_->at(l), stream_), "event record");
+        }
+        for (int l = 0; l <= L; l++) {
+            hip_check(hipEventSynchronize(events_->at(l)), "event wait");
+            comps[l].assign(nb, nullptr);
+            std::vector<uint64_t> at(nb, off[l]);
+            for (int p = 1; p < nb; p++) at[p] = at[p - 1] + m.level_sizes[l][p - 1];
+            parallel_for(nb, default_threads(), [&](int p) {
+                const uint32_t bytes = m.level_sizes[l][p];
+                comps[l][p] = static_cast<uint8_t *>(std::malloc(bytes ? bytes : 1));
+                std::memcpy(comps[l][p], h + at[p], bytes);
+            });
             m.stopping_indices.push_back(compressor_.compress_level(comps[l], m.level_sizes[l]));
         }
+        const auto t2 = clock::now();
         m.level_num = writer_.write_level_components(comps, m.level_sizes);
         const auto bytes = m.serialize_bytes();
         writer_.write_metadata(bytes.data(), (uint32_t)bytes.size());
         for (auto &lv : comps)
             for (uint8_t *b : lv) std::free(b);
+        const auto t3 = clock::now();
+        times_ = {std::chrono::duration<double>(t1 - t0).count(), std::chrono::duration<double>(t2 - t1).count(),
+                  std::chrono::duration<double>(t3 - t2).count()};
     }
     prodm_mdr_plan *plan() const { return plan_ ? plan_->get() : nullptr; }
+    // seconds of the last refactor_device: device stages (to the level
+    // results), copy-out + lossless stage, file writes
+    struct Times {
+        double device = 0, host = 0, write = 0;
+    };
+    Times times() const { return times_; }
 
   private:
+    // grow-only pinned host buffer and per-level events
+    struct Pinned {
+        uint8_t *p = nullptr;
+        uint64_t cap = 0;
+        uint8_t *get(uint64_t n) {
+            if (n > cap) {
+                if (p) (void)hipHostFree(p);
+                p = nullptr;
+                hip_check(hipHostMalloc((void **)&p, n ? n : 1, hipHostMallocDefault), "hipHostMalloc");
+                cap = n;
+            }
+            return p;
+        }
+        ~Pinned() {
+            if (p) (void)hipHostFree(p);
+        }
+    };
+    struct Events {
+        std::vector<hipEvent_t> ev;
+        void ensure(size_t n) {
+            while (ev.size() < n) {
+                hipEvent_t e;
+                hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event create");
+                ev.push_back(e);
+            }
+        }
+        hipEvent_t at(size_t i) const { return ev[i]; }
+        ~Events() {
+            for (auto e : ev) (void)hipEventDestroy(e);
+        }
+    };
+    std::shared_ptr<Pinned> pinned_ = std::make_shared<Pinned>();
+    std::shared_ptr<Events> events_ = std::make_shared<Events>();
+    Times times_;
+
     void prepare(const std::vector<uint32_t> &dims, int L, int nb) {
         if (plan_ && dims == dims_ && L == L_ && nb == nb_) return;
         plan_.reset();

@@ -63,6 +63,9 @@ static int run(const std::string &file, const std::vector<uint32_t> &dims, int L
             grouped ? PRODM_ENC_GROUPED : PRODM_ENC_NEGABINARY, AdaptiveLevelCompressor(64), writer, squared,
             hier ? PRODM_DECOMP_HIERARCHICAL : PRODM_DECOMP_ORTHOGONAL);
         r.refactor(data.data(), dims, L, nb);
+        const auto t = r.times();
+        std::printf("Stages: device %.6fs (upload + GPU stages), copy-out + lossless %.6fs, files %.6fs\n", t.device,
+                    t.host, t.write);
     }
     const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::printf("Refactor time: %.6fs (%s, %s)\n", s, composed ? "composed GPU components" : "device-resident",
