@@ -102,7 +102,8 @@ class DefaultLevelCompressor {
     void decompress_level(std::vector<const uint8_t *> &streams, const std::vector<uint32_t> &sizes,
                           uint8_t start, uint8_t k, uint8_t /*stopping_index*/) {
         std::vector<uint8_t *> out(k, nullptr);
-        parallel_for(k, threads_, [&](int i) { zstd::decompress(streams[i], sizes[start + i], &out[i]); });
+        raw_.assign(k, 0);
+        parallel_for(k, threads_, [&](int i) { raw_[i] = zstd::decompress(streams[i], sizes[start + i], &out[i]); });
         for (int i = 0; i < k; i++) {
             buffer_.push_back(out[i]);
             streams[i] = out[i];
@@ -113,10 +114,13 @@ class DefaultLevelCompressor {
         buffer_.clear();
     }
     void print() const { std::printf("default level compressor (ZSTD, %d threads)\n", threads_); }
+    // byte sizes of the streams the last decompress_level returned
+    const std::vector<uint32_t> &decompressed_sizes() const { return raw_; }
 
   private:
     int threads_;
     std::vector<uint8_t *> buffer_;
+    std::vector<uint32_t> raw_;
 };
 
 class AdaptiveLevelCompressor {
@@ -153,9 +157,10 @@ class AdaptiveLevelCompressor {
     void decompress_level(std::vector<const uint8_t *> &streams, const std::vector<uint32_t> &sizes,
                           uint8_t start, uint8_t k, uint8_t stopping_index) {
         std::vector<uint8_t *> out(k, nullptr);
+        raw_.assign(k, 0);
         parallel_for(k, threads_, [&](int i) {
             const int p = start + i;
-            if (p <= stopping_index || p >= latter_) zstd::decompress(streams[i], sizes[p], &out[i]);
+            raw_[i] = (p <= stopping_index || p >= latter_) ? zstd::decompress(streams[i], sizes[p], &out[i]) : sizes[p];
         });
         for (int i = 0; i < k; i++)
             if (out[i]) {
@@ -168,10 +173,13 @@ class AdaptiveLevelCompressor {
         buffer_.clear();
     }
     void print() const { std::printf("adaptive level compressor (ZSTD, latter %d, %d threads)\n", latter_, threads_); }
+    // byte sizes of the streams the last decompress_level returned
+    const std::vector<uint32_t> &decompressed_sizes() const { return raw_; }
 
   private:
     int latter_, threads_;
     std::vector<uint8_t *> buffer_;
+    std::vector<uint32_t> raw_;
 };
 
 }  // namespace prodm

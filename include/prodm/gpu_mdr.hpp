@@ -54,7 +54,8 @@ class Plan {
 template <class T, class Compressor, class Writer>
 class GpuRefactor {
   public:
-    // encoder: PRODM_ENC_NEGABINARY (ProDM's `negabinary = true`) or PRODM_ENC_GROUPED;
+    // encoder: PRODM_ENC_NEGABINARY (ProDM's `negabinary = true`), PRODM_ENC_GROUPED
+    // or PRODM_ENC_PERBIT;
     // decomposer: PRODM_DECOMP_ORTHOGONAL or PRODM_DECOMP_HIERARCHICAL.
     GpuRefactor(int encoder, Compressor compressor, Writer writer, bool collect_squared = false,
                 int decomposer = PRODM_DECOMP_ORTHOGONAL)
@@ -89,7 +90,7 @@ class GpuRefactor {
         const bool grouped = encoder_ == PRODM_ENC_GROUPED;
         MdrMetadata<T> m;
         m.dims = dims;
-        m.negabinary = !grouped;
+        m.negabinary = encoder_ == PRODM_ENC_NEGABINARY;
         std::vector<std::vector<uint8_t *>> comps(L + 1);
         m.level_sizes.assign(L + 1, {});
         // plane layout (level info synchronizes: the device stages are done)
@@ -125,11 +126,13 @@ class GpuRefactor {
         events_->ensure(L + 1);
         for (int l = 0; l <= L; l++) {
             uint8_t *dst = h + off[l];
-            if (grouped) {
-                const uint32_t nblk = (uint32_t)(head[l] - 4);
-                std::memcpy(dst, &nblk, 4);
-                hip_check(hipMemcpyAsync(dst + 4, rb[l], nblk, hipMemcpyDeviceToHost, stream_), "rb copy");
-                dst += head[l];
+            if (encoder_ != PRODM_ENC_NEGABINARY) {  // planes of their own sizes (GroupedBP, PerBit)
+                if (grouped) {
+                    const uint32_t nblk = (uint32_t)(head[l] - 4);
+                    std::memcpy(dst, &nblk, 4);
+                    hip_check(hipMemcpyAsync(dst + 4, rb[l], nblk, hipMemcpyDeviceToHost, stream_), "rb copy");
+                    dst += head[l];
+                }
                 for (int p = 0; p < nb; p++) {
                     const uint64_t bytes = m.level_sizes[l][p] - (p == 0 ? head[l] : 0);
                     if (bytes) hip_check(hipMemcpyAsync(dst, src[l][p], bytes, hipMemcpyDeviceToHost, stream_), "plane copy");
@@ -247,11 +250,15 @@ class GpuRefactor {
 template <class T, class Compressor, class SizeInterpreter, class ErrorEstimator, class Retriever>
 class GpuReconstructor {
   public:
-    // decomposer: the one the files were refactored with (ProDM's metadata
-    // does not record it)
+    // decomposer: the one the files were refactored with; encoder: the
+    // bitplane encoder, or -1 for the metadata's flag (negabinary, else
+    // grouped). ProDM's metadata records neither the decomposer nor which
+    // non-negabinary encoder wrote the streams: PerBit files need
+    // PRODM_ENC_PERBIT here.
     GpuReconstructor(Compressor compressor, SizeInterpreter interpreter, Retriever retriever,
-                     int decomposer = PRODM_DECOMP_ORTHOGONAL)
-        : compressor_(compressor), interpreter_(interpreter), retriever_(retriever), decomposer_(decomposer) {}
+                     int decomposer = PRODM_DECOMP_ORTHOGONAL, int encoder = -1)
+        : compressor_(compressor), interpreter_(interpreter), retriever_(retriever), decomposer_(decomposer),
+          encoder_(encoder) {}
 
     void load_metadata() {
         size_t size = 0;
@@ -262,8 +269,9 @@ class GpuReconstructor {
         const int nb = (int)meta_.level_sizes[0].size();
         for (const auto &s : meta_.level_sizes)
             if ((int)s.size() != nb) fatal("reconstruct: levels with different plane counts");
-        plan_ = std::make_shared<Plan>(meta_.dims, L, nb, dtype_of<T>(),
-                                       meta_.negabinary ? PRODM_ENC_NEGABINARY : PRODM_ENC_GROUPED);
+        const int enc = plan_encoder();
+        if ((enc == PRODM_ENC_NEGABINARY) != (bool)meta_.negabinary) fatal("reconstruct: encoder does not match the metadata");
+        plan_ = std::make_shared<Plan>(meta_.dims, L, nb, dtype_of<T>(), enc);
         check(prodm_mdr_plan_set_decomposer(plan_->get(), decomposer_), "plan decomposer");
         for (int l = 0; l <= L; l++) {
             const double b = meta_.negabinary ? (double)(meta_.level_error_bounds[l] / 4) : (double)meta_.level_error_bounds[l];
@@ -304,7 +312,8 @@ class GpuReconstructor {
             compressor_.decompress_level(comps[l], meta_.level_sizes[l], prev[l], (uint8_t)k, meta_.stopping_indices[l]);
             for (int i = 0; i < k; i++) {
                 const int p = prev[l] + i;
-                const uint64_t n = raw_size(l, p, comps[l][i]);
+                const uint64_t n = plan_encoder() == PRODM_ENC_PERBIT ? compressor_.decompressed_sizes()[i]
+                                                                      : raw_size(l, p, comps[l][i]);
                 check(prodm_mdr_load_plane_stream(plan_->get(), l, p, comps[l][i], n, stream_), "load plane");
             }
             compressor_.decompress_release();
@@ -321,6 +330,9 @@ class GpuReconstructor {
     size_t get_retrieved_size() const { return retriever_.get_retrieved_size(); }
 
   private:
+    int plan_encoder() const {
+        return encoder_ >= 0 ? encoder_ : meta_.negabinary ? PRODM_ENC_NEGABINARY : PRODM_ENC_GROUPED;
+    }
     // Byte size of a decompressed plane: fixed for negabinary; GroupedBP
     // plane sizes follow from the recording planes of stream 0.
     uint64_t raw_size(int l, int p, const uint8_t *stream) {
@@ -347,6 +359,7 @@ class GpuReconstructor {
     SizeInterpreter interpreter_;
     Retriever retriever_;
     int decomposer_;
+    int encoder_;
     MdrMetadata<T> meta_;
     std::shared_ptr<Plan> plan_;
     std::vector<uint8_t> held_;

@@ -34,6 +34,7 @@ extern "C" {
 #define PRODM_F64 1
 #define PRODM_ENC_NEGABINARY 0
 #define PRODM_ENC_GROUPED 1
+#define PRODM_ENC_PERBIT 2
 
 typedef void *prodm_stream_t;
 
@@ -213,7 +214,11 @@ int prodm_grouped_level_errors_f32(const float *d_x, uint64_t n, int32_t exp, in
  *      (ComposedReconstructor.hpp:349-420: decode, reposition, recompose).
  *      Interleave is fused into the decomposition (coefficients are written
  *      straight into per-level buffers) and reposition into the
- *      recomposition. Lossless compression and file I/O stay on the host.  */
+ *      recomposition. Lossless compression and file I/O stay on the host.
+ *      encoder: PRODM_ENC_NEGABINARY, PRODM_ENC_GROUPED or PRODM_ENC_PERBIT
+ *      (PerBitBPEncoder: encoded on the device; prodm_mdr_retrieve decodes
+ *      its streams on the host, see prodm_perbit_decode_f32, and has no
+ *      graph form).                                                         */
 typedef struct prodm_mdr_plan prodm_mdr_plan;
 int prodm_mdr_plan_create(prodm_mdr_plan **out, const uint32_t *dims, int ndims, int target_level, int num_bitplanes,
                           int dtype, int encoder);
@@ -227,7 +232,8 @@ int prodm_mdr_level_info(prodm_mdr_plan *plan, int level, double *max_abs, int32
  * prodm_negabinary_level_errors_f32); synchronizes                        */
 int prodm_mdr_level_errors(prodm_mdr_plan *plan, int level, double *h_out, prodm_stream_t stream);
 /* device pointer and byte size of plane p of level l (reference stream
- * bytes without the GroupedBP stream-0 header); synchronizes for GroupedBP */
+ * bytes without the GroupedBP stream-0 header); synchronizes for GroupedBP
+ * and PerBit                                                              */
 int prodm_mdr_plane(prodm_mdr_plan *plan, int level, int plane, const void **d_ptr, uint64_t *bytes);
 /* GroupedBP: device pointer to the rb bytes of a level                   */
 int prodm_mdr_level_rb(prodm_mdr_plan *plan, int level, const uint8_t **d_rb, uint64_t *nblocks);

@@ -2,14 +2,15 @@
 // (counterpart of ProDM's test/test_mdr_reconstructor.cpp:44-88).
 //
 //   prodm_reconstruct <original data file> <num tolerances> <tolerances...>
-//                     [--f64] [--l2] [--hierarchical] [--in DIR]
+//                     [--f64] [--l2] [--hierarchical] [--perbit] [--in DIR]
 //
 // Reads DIR/metadata.bin (default DIR refactored_data). For each tolerance:
 // plans with the greedy interpreter (L-infinity: MaxErrorEstimatorOB;
 // --l2: S-norm estimator with the negabinary greedy interpreter, needs files
 // written with --squared; --hierarchical: files written with the hierarchical
 // basis, planned with MaxErrorEstimatorHB and the sign-excluding greedy
-// planner as ProDM's driver does), reads and decodes the new planes,
+// planner as ProDM's driver does; --perbit: files written with the PerBit
+// encoder, which ProDM's metadata does not record), reads and decodes the new planes,
 // recomposes, and prints the retrieved bytes and the error against the
 // original.
 #include <chrono>
@@ -58,7 +59,9 @@ static void evaluate(const std::vector<T> &u, const std::vector<double> &tols, R
 }
 
 template <class T>
-static int run(const std::string &file, const std::vector<double> &tols, bool l2, bool hier, const std::string &dir) {
+static int run(const std::string &file, const std::vector<double> &tols, bool l2, bool hier, bool perbit,
+               const std::string &dir) {
+    const int enc = perbit ? PRODM_ENC_PERBIT : -1;
     size_t n = 0;
     const auto u = read_raw<T>(file, n);
     size_t msize = 0;
@@ -77,20 +80,21 @@ static int run(const std::string &file, const std::vector<double> &tols, bool l2
         GpuReconstructor<T, AdaptiveLevelCompressor, SignExcludeGreedyBasedSizeInterpreter<E>, E,
                          ConcatLevelFileRetriever>
             r(AdaptiveLevelCompressor(64), SignExcludeGreedyBasedSizeInterpreter<E>(E()), retriever,
-              PRODM_DECOMP_HIERARCHICAL);
+              PRODM_DECOMP_HIERARCHICAL, enc);
         r.load_metadata();
         evaluate(u, tols, r);
     } else if (l2) {
         using E = SNormErrorEstimator<T>;
         const int L = (int)files.size() - 1;
         GpuReconstructor<T, AdaptiveLevelCompressor, NegaBinaryGreedyBasedSizeInterpreter<E>, E, ConcatLevelFileRetriever>
-            r(AdaptiveLevelCompressor(64), NegaBinaryGreedyBasedSizeInterpreter<E>(E(3, L, 0)), retriever);
+            r(AdaptiveLevelCompressor(64), NegaBinaryGreedyBasedSizeInterpreter<E>(E(3, L, 0)), retriever,
+              PRODM_DECOMP_ORTHOGONAL, enc);
         r.load_metadata();
         evaluate(u, tols, r);
     } else {
         using E = MaxErrorEstimatorOB<T>;
         GpuReconstructor<T, AdaptiveLevelCompressor, GreedyBasedSizeInterpreter<E>, E, ConcatLevelFileRetriever> r(
-            AdaptiveLevelCompressor(64), GreedyBasedSizeInterpreter<E>(E(3)), retriever);
+            AdaptiveLevelCompressor(64), GreedyBasedSizeInterpreter<E>(E(3)), retriever, PRODM_DECOMP_ORTHOGONAL, enc);
         r.load_metadata();
         evaluate(u, tols, r);
     }
@@ -108,15 +112,16 @@ int main(int argc, char **argv) {
     const int nt = std::atoi(argv[a++]);
     std::vector<double> tols;
     for (int i = 0; i < nt && a < argc; i++) tols.push_back(std::atof(argv[a++]));
-    bool f64 = false, l2 = false, hier = false;
+    bool f64 = false, l2 = false, hier = false, perbit = false;
     std::string dir = "refactored_data";
     for (; a < argc; a++) {
         const std::string o = argv[a];
         if (o == "--f64") f64 = true;
         else if (o == "--l2") l2 = true;
         else if (o == "--hierarchical") hier = true;
+        else if (o == "--perbit") perbit = true;
         else if (o == "--in" && a + 1 < argc) dir = argv[++a];
         else fatal("unknown option " + o);
     }
-    return f64 ? run<double>(file, tols, l2, hier, dir) : run<float>(file, tols, l2, hier, dir);
+    return f64 ? run<double>(file, tols, l2, hier, perbit, dir) : run<float>(file, tols, l2, hier, perbit, dir);
 }

@@ -2,13 +2,15 @@
 // (counterpart of ProDM's test/test_mdr_refactor.cpp:34-86).
 //
 //   prodm_refactor <data file> <target level> <num bitplanes> <num dims> <dims...>
-//                  [--f64] [--grouped] [--composed] [--squared] [--hierarchical] [--out DIR]
+//                  [--f64] [--grouped | --perbit] [--composed] [--squared] [--hierarchical] [--out DIR]
 //
 // Writes DIR/metadata.bin and DIR/level_<l>.bin (default DIR refactored_data).
 // Default: float, negabinary encoder, AdaptiveLevelCompressor(64), the
 // device-resident GpuRefactor; --composed drives the GPU components through
 // prodm::ComposedRefactor instead (same files). --hierarchical uses the
-// hierarchical basis (ProDM's drivers use MGARDHierarchicalDecomposer).
+// hierarchical basis (ProDM's drivers use MGARDHierarchicalDecomposer);
+// --perbit the PerBit encoder (with --hierarchical, the GE applications'
+// pairing).
 #include <sys/stat.h>
 
 #include <chrono>
@@ -31,8 +33,8 @@ static std::vector<T> read_raw(const std::string &path, size_t n) {
 }
 
 template <class T, class S>
-static int run(const std::string &file, const std::vector<uint32_t> &dims, int L, int nb, bool grouped, bool composed,
-               bool squared, bool hier, const std::string &dir) {
+static int run(const std::string &file, const std::vector<uint32_t> &dims, int L, int nb, bool grouped, bool perbit,
+               bool composed, bool squared, bool hier, const std::string &dir) {
     const auto data = read_raw<T>(file, product(dims));
     std::vector<std::string> files;
     for (int l = 0; l <= L; l++) files.push_back(dir + "/level_" + std::to_string(l) + ".bin");
@@ -44,14 +46,17 @@ static int run(const std::string &file, const std::vector<uint32_t> &dims, int L
             ComposedRefactor<T, decltype(decomposer), GpuDirectInterleaver<T>, decltype(encoder),
                              AdaptiveLevelCompressor, decltype(collector), ConcatLevelFileWriter>
                 r{decomposer, GpuDirectInterleaver<T>(ctx), encoder, AdaptiveLevelCompressor(64), collector, writer};
-            r.negabinary = !grouped;
+            r.negabinary = !grouped && !perbit;
             r.refactor(data.data(), dims, L, nb);
         };
         auto go = [&](auto encoder, auto collector) {
             if (hier) go2(GpuMGARDHierarchicalDecomposer<T>(ctx), encoder, collector);
             else go2(GpuMGARDOrthogonalDecomposer<T>(ctx), encoder, collector);
         };
-        if (grouped) {
+        if (perbit) {
+            if (squared) go(GpuPerBitBPEncoder<T, S>(ctx), GpuSquaredErrorCollector<T>(ctx));
+            else go(GpuPerBitBPEncoder<T, S>(ctx), MaxErrorCollector<T>());
+        } else if (grouped) {
             if (squared) go(GpuGroupedBPEncoder<T, S>(ctx), GpuSquaredErrorCollector<T>(ctx));
             else go(GpuGroupedBPEncoder<T, S>(ctx), MaxErrorCollector<T>());
         } else {
@@ -60,7 +65,7 @@ static int run(const std::string &file, const std::vector<uint32_t> &dims, int L
         }
     } else {
         GpuRefactor<T, AdaptiveLevelCompressor, ConcatLevelFileWriter> r(
-            grouped ? PRODM_ENC_GROUPED : PRODM_ENC_NEGABINARY, AdaptiveLevelCompressor(64), writer, squared,
+            perbit ? PRODM_ENC_PERBIT : grouped ? PRODM_ENC_GROUPED : PRODM_ENC_NEGABINARY, AdaptiveLevelCompressor(64), writer, squared,
             hier ? PRODM_DECOMP_HIERARCHICAL : PRODM_DECOMP_ORTHOGONAL);
         r.refactor(data.data(), dims, L, nb);
         const auto t = r.times();
@@ -69,14 +74,14 @@ static int run(const std::string &file, const std::vector<uint32_t> &dims, int L
     }
     const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::printf("Refactor time: %.6fs (%s, %s)\n", s, composed ? "composed GPU components" : "device-resident",
-                grouped ? "grouped" : "negabinary");
+                perbit ? "per-bit" : grouped ? "grouped" : "negabinary");
     return 0;
 }
 
 int main(int argc, char **argv) {
     if (argc < 6) {
         std::fprintf(stderr, "usage: %s <file> <target level> <bitplanes> <ndims> <dims...> [--f64] [--grouped] "
-                             "[--composed] [--squared] [--out DIR]\n", argv[0]);
+                             "[--perbit] [--composed] [--squared] [--hierarchical] [--out DIR]\n", argv[0]);
         return 2;
     }
     int a = 1;
@@ -86,25 +91,26 @@ int main(int argc, char **argv) {
     const int nd = std::atoi(argv[a++]);
     std::vector<uint32_t> dims;
     for (int i = 0; i < nd && a < argc; i++) dims.push_back((uint32_t)std::atoi(argv[a++]));
-    bool f64 = false, grouped = false, composed = false, squared = false, hier = false;
+    bool f64 = false, grouped = false, perbit = false, composed = false, squared = false, hier = false;
     std::string dir = "refactored_data";
     for (; a < argc; a++) {
         const std::string o = argv[a];
         if (o == "--f64") f64 = true;
         else if (o == "--grouped") grouped = true;
+        else if (o == "--perbit") perbit = true;
         else if (o == "--composed") composed = true;
         else if (o == "--squared") squared = true;
         else if (o == "--hierarchical") hier = true;
         else if (o == "--out" && a + 1 < argc) dir = argv[++a];
         else fatal("unknown option " + o);
     }
-    if (!grouped && (nb & 1)) {
+    if (!grouped && !perbit && (nb & 1)) {
         nb += 1;  // negabinary needs an even plane count (test_mdr_refactor.cpp:40-43)
         std::printf("Change to %d bitplanes for negabinary encoding\n", nb);
     }
     const int maxnb = f64 ? 64 : 32;
     if (nb > maxnb) nb = maxnb;
     mkdir(dir.c_str(), 0755);
-    return f64 ? run<double, uint64_t>(file, dims, L, nb, grouped, composed, squared, hier, dir)
-               : run<float, uint32_t>(file, dims, L, nb, grouped, composed, squared, hier, dir);
+    return f64 ? run<double, uint64_t>(file, dims, L, nb, grouped, perbit, composed, squared, hier, dir)
+               : run<float, uint32_t>(file, dims, L, nb, grouped, perbit, composed, squared, hier, dir);
 }

@@ -13,7 +13,7 @@ LIB_PATH = os.environ.get("PRODM_MDR_LIB") or os.path.join(HERE, "libprodm_mdr.s
 
 F32, F64 = 0, 1
 ORTHOGONAL, HIERARCHICAL = 0, 1
-NEGABINARY, GROUPED = 0, 1
+NEGABINARY, GROUPED, PERBIT = 0, 1, 2
 
 _lib = None
 

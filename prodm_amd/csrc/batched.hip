@@ -157,16 +157,23 @@ int run_launch(const GroupLaunch &L, hipStream_t s) {
 }
 
 template <class T> int grouped_encode_fields(prodm_mdr_batch *b, hipStream_t s) {
-    // GroupedBP: the plan's per-level encode (rb histogram, scan, compaction)
+    // GroupedBP and PerBit: the plan's per-level encode (count or rb
+    // histogram, scan, write)
     using S = typename Traits<T>::Stream;
     for (prodm_mdr_plan *p : b->plans)
         for (int l = p->g.L; l >= 0; l--) {
             const T *x = static_cast<const T *>(p->levels) + p->level_off[l];
             S *planes = static_cast<S *>(p->planes) + p->plane_off[l];
-            if (int rc = launch_gb_encode<T>(x, (uint32_t)p->g.elems[l], 0, static_cast<const KeyOf<T> *>(p->keys) + l,
-                                             p->nb, 0, p->rb + p->rb_off[l], planes, p->plane_stride[l],
-                                             p->words + (size_t)l * p->nb, p->gbws, p->gbws_bytes, s))
-                return rc;
+            const KeyOf<T> *key = static_cast<const KeyOf<T> *>(p->keys) + l;
+            const uint32_t n = (uint32_t)p->g.elems[l];
+            const int rc =
+                b->encoder == PRODM_ENC_PERBIT
+                    ? launch_pb_encode<T>(x, n, 0, key, p->nb, 0, reinterpret_cast<uint64_t *>(planes),
+                                          p->plane_stride[l] * sizeof(T) / 8, p->words + (size_t)l * p->nb, p->gbws,
+                                          p->gbws_bytes, s)
+                    : launch_gb_encode<T>(x, n, 0, key, p->nb, 0, p->rb + p->rb_off[l], planes, p->plane_stride[l],
+                                          p->words + (size_t)l * p->nb, p->gbws, p->gbws_bytes, s);
+            if (rc) return rc;
         }
     return kOk;
 }
@@ -174,7 +181,7 @@ template <class T> int grouped_encode_fields(prodm_mdr_batch *b, hipStream_t s) 
 int run_all(prodm_mdr_batch *b, hipStream_t s) {
     for (const GroupLaunch &L : b->launches)
         if (int rc = run_launch(L, s)) return rc;
-    if (b->encoder == PRODM_ENC_GROUPED)
+    if (b->encoder == PRODM_ENC_GROUPED || b->encoder == PRODM_ENC_PERBIT)
         return b->dtype == PRODM_F64 ? grouped_encode_fields<double>(b, s) : grouped_encode_fields<float>(b, s);
     return kOk;
 }

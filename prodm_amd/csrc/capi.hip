@@ -308,6 +308,10 @@ template <class T> static int plan_refactor(prodm_mdr_plan *p, const T *in, hipS
         if (p->encoder == PRODM_ENC_NEGABINARY)
             return launch_nb_encode<T>(sink.level[l], (uint32_t)g.elems[l], 0, keys + l, p->nb, planes,
                                        p->plane_stride[l], es);
+        if (p->encoder == PRODM_ENC_PERBIT)
+            return launch_pb_encode<T>(sink.level[l], (uint32_t)g.elems[l], 0, keys + l, p->nb, 0,
+                                       reinterpret_cast<uint64_t *>(planes), p->plane_stride[l] * sizeof(T) / 8,
+                                       p->words + (size_t)l * p->nb, p->gbws, p->gbws_bytes, es);
         return launch_gb_encode<T>(sink.level[l], (uint32_t)g.elems[l], 0, keys + l, p->nb, 0, p->rb + p->rb_off[l],
                                    planes, p->plane_stride[l], p->words + (size_t)l * p->nb, p->gbws, p->gbws_bytes, es);
     };
@@ -342,6 +346,51 @@ template <class T> static int plan_refactor(prodm_mdr_plan *p, const T *in, hipS
     return kOk;
 }
 
+// PerBit planes [0, k[l]) of every level decoded on the host (the streams
+// have no index of where an element's bits begin) into the level buffers.
+// Synchronizes the stream.
+template <class T> static int perbit_decode_levels(prodm_mdr_plan *p, const uint8_t *k, hipStream_t s) {
+    const LevelGeom &g = p->g;
+    PRODM_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> sizes((size_t)(g.L + 1) * p->nb);
+    PRODM_HIP(hipMemcpy(sizes.data(), p->words, sizes.size() * 4, hipMemcpyDeviceToHost));
+    prodm_perbit_state *st = nullptr;
+    if (int rc = prodm_perbit_state_create(&st)) return rc;
+    int rc = kOk;
+    std::vector<T> vals;
+    std::vector<std::vector<uint64_t>> words;
+    for (int l = 0; l <= g.L && !rc; l++) {
+        const uint64_t n = g.elems[l];
+        double mx;
+        int32_t e;
+        if ((rc = prodm_mdr_level_info(p, l, &mx, &e, nullptr))) break;
+        words.assign(k[l], {});
+        std::vector<const uint64_t *> ptrs(k[l] ? k[l] : 1);
+        const char *base = static_cast<const char *>(p->planes) + p->elem * p->plane_off[l];
+        for (int i = 0; i < k[l]; i++) {
+            const uint32_t bytes = sizes[(size_t)l * p->nb + i];
+            words[i].resize(bytes / 8 + 1);
+            if (bytes)
+                if (hipMemcpy(words[i].data(), base + p->elem * p->plane_stride[l] * i, bytes, hipMemcpyDeviceToHost) !=
+                    hipSuccess) { set_error("retrieve: plane copy"); rc = kErrHip; break; }
+            ptrs[i] = words[i].data();
+        }
+        if (rc) break;
+        vals.resize(n ? n : 1);
+        std::vector<uint32_t> sz(k[l] ? k[l] : 1);
+        for (int i = 0; i < k[l]; i++) sz[i] = sizes[(size_t)l * p->nb + i];
+        rc = sizeof(T) == 4 ? prodm_perbit_decode_f32(st, l, ptrs.data(), sz.data(), (uint32_t)n, e, 0, k[l],
+                                                      reinterpret_cast<float *>(vals.data()))
+                            : prodm_perbit_decode_f64(st, l, ptrs.data(), sz.data(), (uint32_t)n, e, 0, k[l],
+                                                      reinterpret_cast<double *>(vals.data()));
+        if (!rc && n &&
+            hipMemcpy(static_cast<T *>(p->levels) + p->level_off[l], vals.data(), n * sizeof(T), hipMemcpyHostToDevice) !=
+                hipSuccess) { set_error("retrieve: level upload"); rc = kErrHip; }
+    }
+    prodm_perbit_state_destroy(st);
+    return rc;
+}
+
 template <class T> static int plan_retrieve(prodm_mdr_plan *p, const uint8_t *k, T *out, hipStream_t s) {
     using S = typename Traits<T>::Stream;
     using K = KeyOf<T>;
@@ -353,6 +402,12 @@ template <class T> static int plan_retrieve(prodm_mdr_plan *p, const uint8_t *k,
         if (k[l] > p->nb) { set_error("retrieve: more planes than encoded"); return kErrArg; }
     StageTimer *tm = p->timer.on ? &p->timer : nullptr;
     if (tm) tm->stream = s;
+    if (p->encoder == PRODM_ENC_PERBIT) {
+        if (p->capturing) { set_error("retrieve: PerBit decodes on the host and has no graph form"); return kErrUnsupported; }
+        const uint64_t st[3] = {(uint64_t)g.dims[1] * g.dims[2], g.dims[2], 1};
+        if (int rc = perbit_decode_levels<T>(p, k, s)) return rc;
+        return mgard_recompose<T>(src, g, 1, out, st, p->ws, p->ws_bytes, s, tm, p->hier != 0);
+    }
     auto decode_level = [&](int l, hipStream_t ds) -> int {
         StageScope sc(tm, l == g.L ? kStDecodeTop : kStDecodeRest);
         S *planes = static_cast<S *>(p->planes) + p->plane_off[l];
@@ -634,7 +689,10 @@ int prodm_mdr_plan_create(prodm_mdr_plan **out, const uint32_t *dims, int ndims,
     if (!out) { set_error("plan: null output"); return kErrArg; }
     *out = nullptr;
     if (dtype != PRODM_F32 && dtype != PRODM_F64) { set_error("plan: bad dtype"); return kErrArg; }
-    if (encoder != PRODM_ENC_NEGABINARY && encoder != PRODM_ENC_GROUPED) { set_error("plan: bad encoder"); return kErrArg; }
+    if (encoder != PRODM_ENC_NEGABINARY && encoder != PRODM_ENC_GROUPED && encoder != PRODM_ENC_PERBIT) {
+        set_error("plan: bad encoder");
+        return kErrArg;
+    }
     const int B = dtype == PRODM_F64 ? 64 : 32;
     if (nb < 1 || nb > B) { set_error("plan: num_bitplanes out of range"); return kErrArg; }
     if (encoder == PRODM_ENC_NEGABINARY && (nb & 1)) {
@@ -658,6 +716,7 @@ int prodm_mdr_plan_create(prodm_mdr_plan **out, const uint32_t *dims, int ndims,
         lo += (g.elems[l] + 63) & ~63ull;
         const uint64_t nblk = (g.elems[l] + B - 1) / B;
         uint64_t stride = encoder == PRODM_ENC_NEGABINARY ? nblk : 2 * nblk;
+        if (encoder == PRODM_ENC_PERBIT) stride = pb_stream_words((uint32_t)g.elems[l]) * 8 / sw;  // 64-bit words
         stride = (stride + 63) & ~63ull;  // 256-byte aligned planes
         p->plane_stride[l] = stride;
         p->plane_off[l] = po;
@@ -672,6 +731,16 @@ int prodm_mdr_plan_create(prodm_mdr_plan **out, const uint32_t *dims, int ndims,
     if (!rc) rc = plan_alloc(&p->levels, lo * p->elem, p->device_bytes);
     if (!rc) rc = plan_alloc(&p->keys, 8 * (g.L + 1), p->device_bytes);
     if (!rc) rc = plan_alloc(&p->planes, po * sw, p->device_bytes);
+    if (!rc && encoder == PRODM_ENC_PERBIT) {
+        rc = plan_alloc((void **)&p->words, 4ull * (g.L + 1) * nb, p->device_bytes);
+        size_t need = 0;
+        for (int l = 0; l <= g.L; l++) {
+            const uint32_t n = (uint32_t)g.elems[l];
+            need = std::max(need, dtype == PRODM_F64 ? pb_workspace_bytes<double>(n, nb) : pb_workspace_bytes<float>(n, nb));
+        }
+        p->gbws_bytes = need;
+        if (!rc) rc = plan_alloc(&p->gbws, need, p->device_bytes);
+    }
     if (!rc && encoder == PRODM_ENC_GROUPED) {
         rc = plan_alloc((void **)&p->rb, ro, p->device_bytes);
         if (!rc) rc = plan_alloc((void **)&p->words, 4ull * (g.L + 1) * nb, p->device_bytes);
@@ -753,7 +822,7 @@ int prodm_mdr_plane(prodm_mdr_plan *p, int level, int plane, const void **d_ptr,
             uint32_t w = 0;
             PRODM_HIP(hipDeviceSynchronize());
             PRODM_HIP(hipMemcpy(&w, p->words + (size_t)level * p->nb + plane, 4, hipMemcpyDeviceToHost));
-            *bytes = (uint64_t)w * p->elem;
+            *bytes = p->encoder == PRODM_ENC_PERBIT ? (uint64_t)w : (uint64_t)w * p->elem;  // PerBit keeps bytes
         }
     }
     return kOk;
@@ -832,6 +901,11 @@ int prodm_mdr_load_plane_stream(prodm_mdr_plan *p, int level, int plane, const u
     uint64_t head = 0;
     if (p->encoder == PRODM_ENC_NEGABINARY) {
         if (bytes != nblk * p->elem) { set_error("load_plane_stream: negabinary plane size mismatch"); return kErrArg; }
+    } else if (p->encoder == PRODM_ENC_PERBIT) {
+        if (bytes % 8) { set_error("load_plane_stream: PerBit streams are whole 64-bit words"); return kErrArg; }
+        if (bytes > cap) { set_error("load_plane_stream: plane larger than its storage"); return kErrArg; }
+        const uint32_t b32 = (uint32_t)bytes;
+        PRODM_HIP(hipMemcpyAsync(p->words + (size_t)level * p->nb + plane, &b32, 4, hipMemcpyHostToDevice, s));
     } else {
         if (plane == 0) {
             uint32_t hdr = 0;
@@ -884,6 +958,10 @@ int prodm_mdr_refactor_graph(prodm_mdr_plan *p, const void *d_in, prodm_stream_t
 
 int prodm_mdr_retrieve_graph(prodm_mdr_plan *p, const uint8_t *k, void *d_out, prodm_stream_t stream) {
     if (!p || !k || !d_out) { set_error("retrieve_graph: null argument"); return kErrArg; }
+    if (p->encoder == PRODM_ENC_PERBIT) {
+        set_error("retrieve_graph: PerBit decodes on the host and has no graph form");
+        return kErrUnsupported;
+    }
     hipStream_t s = (hipStream_t)stream;
     if (!p->g_ret || p->g_ret_out != d_out || std::memcmp(p->g_ret_k, k, p->g.L + 1) != 0) {
         const bool on = p->timer.on;

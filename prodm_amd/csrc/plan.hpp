@@ -20,7 +20,7 @@ struct prodm_mdr_plan {
     uint64_t plane_stride[33] = {0}; // words per plane
     uint8_t *rb = nullptr;           // GroupedBP recording planes
     uint64_t rb_off[33] = {0};
-    uint32_t *words = nullptr;       // GroupedBP words per plane, (L+1) * nb
+    uint32_t *words = nullptr;       // GroupedBP words / PerBit bytes per plane, (L+1) * nb
     void *signs = nullptr;           // GroupedBP decode signs
     void *gbws = nullptr;
     size_t gbws_bytes = 0;

@@ -11,8 +11,9 @@
 //  4. GPU components against each other: encode -> progressive decode
 //     equals one-shot decode (negabinary, grouped, per-bit); interleave ->
 //     reposition is the identity.
-//  5. The per-bit encoder with the hierarchical decomposer through the
-//     composers meets each requested L-infinity tolerance.
+//  5. The per-bit encoder with the hierarchical decomposer: the composers
+//     and GpuRefactor write byte-identical files; the composed and the
+//     device-resident reconstructions meet each L-infinity tolerance.
 //
 // Usage: test_cpp_pipeline <scratch dir>. Exit status 0 on success.
 #include <sys/stat.h>
@@ -221,9 +222,16 @@ static void hierarchical_path(const std::string &root, const std::vector<float> 
 // the progressive reconstruction meets each L-infinity tolerance.
 static void perbit_path(const std::string &root, const std::vector<float> &u, const std::vector<uint32_t> &dims,
                         int L, int nb) {
-    const std::string a = root + "/composed_pb";
+    const std::string a = root + "/composed_pb", b = root + "/gpu_pb";
     mkdir(a.c_str(), 0755);
+    mkdir(b.c_str(), 0755);
     Context ctx = make_context();
+    {  // the device-resident path with the PerBit encoder writes the same files
+        GpuRefactor<float, AdaptiveLevelCompressor, ConcatLevelFileWriter> r(
+            PRODM_ENC_PERBIT, AdaptiveLevelCompressor(64), ConcatLevelFileWriter(b + "/metadata.bin", level_files(b, L)),
+            false, PRODM_DECOMP_HIERARCHICAL);
+        r.refactor(u.data(), dims, L, nb);
+    }
     {
         ComposedRefactor<float, GpuMGARDHierarchicalDecomposer<float>, GpuDirectInterleaver<float>,
                          GpuPerBitBPEncoder<float, uint32_t>, AdaptiveLevelCompressor, MaxErrorCollector<float>,
@@ -242,7 +250,20 @@ static void perbit_path(const std::string &root, const std::vector<float> &u, co
            GpuPerBitBPEncoder<float, uint32_t>(ctx), AdaptiveLevelCompressor(64), Interp(Est()),
            ConcatLevelFileRetriever(a + "/metadata.bin", level_files(a, L))};
     rc.load_metadata();
+    EXPECT(slurp(a + "/metadata.bin") == slurp(b + "/metadata.bin"), "per-bit metadata differs");
+    for (int l = 0; l <= L; l++)
+        EXPECT(slurp(level_files(a, L)[l]) == slurp(level_files(b, L)[l]), "per-bit level %d file differs", l);
+    GpuReconstructor<float, AdaptiveLevelCompressor, Interp, Est, ConcatLevelFileRetriever> rg(
+        AdaptiveLevelCompressor(64), Interp(Est()), ConcatLevelFileRetriever(b + "/metadata.bin", level_files(b, L)),
+        PRODM_DECOMP_HIERARCHICAL, PRODM_ENC_PERBIT);
+    rg.load_metadata();
     const double R = range_of(u);
+    for (double rel : {1e-1, 1e-3, 1e-5}) {
+        const float *g = rg.progressive_reconstruct(rel * R);
+        const double e = linf(u, g);
+        std::printf("per-bit tol %.1e: gpu err %.3e, retrieved %zu bytes\n", rel, e / R, rg.get_retrieved_size());
+        EXPECT(e <= rel * R, "per-bit gpu error %.3e above %.3e", e, rel * R);
+    }
     // recompose_to_full overwrites the held field (ComposedReconstructor.hpp:123-131),
     // so a coarse-grid result is only reported
     int checked = 0;

@@ -131,3 +131,97 @@ def test_perbit_encode_rejects_bad_arguments(gpu):
     assert lib.prodm_perbit_encode_f32(*args(100, 8, 3)) != 0      # stride below ceil(2 n / 64)
     assert lib.prodm_perbit_encode_f32(*args(100, 8, 4)) == 0
     torch.cuda.synchronize()
+
+
+# ---- the device-resident plan with the PerBit encoder --------------------------------
+def _plan_case(dt, dims, L, nb, enc, decomp, seed):
+    from fields import bumps_field
+    x = torch.from_numpy(bumps_field(dims, seed=seed, dtype=dt)).cuda()
+    p = mdr.Plan(dims, L, nb, TD[dt], enc, decomp)
+    p.refactor(x)
+    torch.cuda.synchronize()
+    return x, p
+
+
+@pytest.mark.parametrize("dt,nb,decomp", [(np.float32, 32, 0), (np.float32, 32, 1), (np.float64, 60, 0),
+                                          (np.float64, 64, 1)])
+def test_plan_perbit_streams_match_oracle(gpu, dt, nb, decomp):
+    """Every level's streams from the plan equal the oracle's PerBit encode
+    of that level's coefficients at the level's exponent."""
+    dims, L = [33, 65, 33], 4
+    _, p = _plan_case(dt, dims, L, nb, capi.PERBIT, decomp, 7)
+    for l in range(L + 1):
+        c = p.coefficients(l).cpu().numpy()
+        _, e, n = p.level_info(l)
+        assert n == c.size
+        ref = ob.pb_encode(c, e, nb) if nb < 64 else None
+        for q in range(nb):
+            s = p.plane_stream(l, q)
+            assert s == p.plane_bytes(l, q)
+            if ref is not None:
+                assert s == ref[q], (l, q)
+        if ref is None:  # 64 planes: decode every plane on the host and compare with truncation
+            y = mdr.PerBitDecoder(TD[dt]).decode(0, [p.plane_stream(l, q) for q in range(nb)], c.size, e, 0, nb)
+            t = np.array([float(int(abs(float(v)) * 2.0 ** (nb - e))) for v in c]) * 2.0 ** (e - nb)
+            assert np.array_equal(np.abs(y), t), l
+
+
+@pytest.mark.parametrize("dt,nb,decomp", [(np.float32, 32, 0), (np.float32, 32, 1), (np.float64, 52, 0)])
+def test_plan_perbit_retrieve_equals_grouped(gpu, dt, nb, decomp):
+    """PerBit and GroupedBP keep the same magnitude bits and signs: a
+    retrieval of the same planes per level is equal value for value. (Where
+    no 1 bit is retrieved, GroupedBP's block sign can make a coefficient
+    -0.0 and PerBit's +0.0, as in ProDM's decoders, so zeros may differ in
+    sign.)"""
+    dims, L = [65, 33, 65], 5
+    x, pp = _plan_case(dt, dims, L, nb, capi.PERBIT, decomp, 11)
+    _, pg = _plan_case(dt, dims, L, nb, capi.GROUPED, decomp, 11)
+    for k in ([nb] * (L + 1), [min(nb, 3 + 4 * l) for l in range(L + 1)], [0] + [nb // 2] * L):
+        a = pp.retrieve(k)
+        b = pg.retrieve(k)
+        torch.cuda.synchronize()
+        assert torch.equal(a, b), k
+
+
+def test_plan_perbit_load_streams_and_errors(gpu):
+    """Streams moved out and loaded into a fresh plan retrieve the same
+    field; level errors are GroupedBP's; graph retrieval is refused."""
+    dims, L, nb = [65, 65, 33], 4, 32
+    x, p = _plan_case(np.float32, dims, L, nb, capi.PERBIT, 0, 3)
+    _, pg = _plan_case(np.float32, dims, L, nb, capi.GROUPED, 0, 3)
+    for l in range(L + 1):  # before any retrieval (which decodes into the level buffers)
+        np.testing.assert_array_equal(p.level_errors(l), pg.level_errors(l))
+    q = mdr.Plan(dims, L, nb, torch.float32, capi.PERBIT)
+    for l in range(L + 1):
+        mx, _, _ = p.level_info(l)
+        q.set_level_max(l, mx)
+        for j in range(nb):
+            q.load_plane_stream(l, j, p.plane_stream(l, j))
+    k = [nb, 30, 24, 18, 12]
+    a, b = p.retrieve(k), q.retrieve(k)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    out = torch.empty_like(x)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        with pytest.raises(capi.ProdmError):
+            p.retrieve_graph(k, out)
+    with pytest.raises(capi.ProdmError):
+        q.load_plane_stream(0, 0, b"\0" * 12)  # not whole 64-bit words
+
+
+def test_batched_perbit_equals_plain(gpu):
+    from prodm_amd.batched import Batch
+    from fields import bumps_field
+    shapes = [([65, 65, 33], 4), ([129, 65], 5), ([33, 33, 33], 3)]
+    xs = [torch.from_numpy(bumps_field(d, seed=50 + i, dtype=np.float32)).cuda() for i, (d, L) in enumerate(shapes)]
+    b = Batch([(x, d, L) for x, (d, L) in zip(xs, shapes)], 32, torch.float32, capi.PERBIT)
+    b.refactor()
+    torch.cuda.synchronize()
+    for f, (d, L) in enumerate(shapes):
+        p = mdr.Plan(d, L, 32, torch.float32, capi.PERBIT)
+        p.refactor(xs[f])
+        torch.cuda.synchronize()
+        for l in range(L + 1):
+            for q in range(32):
+                assert b.plan(f).plane_stream(l, q) == p.plane_stream(l, q), (f, l, q)
