@@ -1088,8 +1088,9 @@ __global__ __launch_bounds__(BlkStage<T>::kTPB) void k_pb_write(const T *__restr
         const uint64_t o = ((uint64_t)readlane_u32((uint32_t)(my_off >> 32), p) << 32) |
                            readlane_u32((uint32_t)my_off, p);
         const uint32_t c = (uint32_t)__popcll(F);
-        const uint32_t flags = __ockl_wfred_add_u32(c);
-        const uint32_t q = (uint32_t)(o & 63) + below + __ockl_wfscan_add_u32(c, false);
+        const uint32_t pre = __ockl_wfscan_add_u32(c, false);
+        const uint32_t flags = readlane_u32(pre + c, 63);  // the wave's total
+        const uint32_t q = (uint32_t)(o & 63) + below + pre;
         // insert each flagged element's sign after its bit, highest first
         R r = (R)D;
         uint64_t ff = F;
