@@ -166,14 +166,16 @@ def test_plan_perbit_streams_match_oracle(gpu, dt, nb, decomp):
             assert np.array_equal(np.abs(y), t), l
 
 
-@pytest.mark.parametrize("dt,nb,decomp", [(np.float32, 32, 0), (np.float32, 32, 1), (np.float64, 52, 0)])
-def test_plan_perbit_retrieve_equals_grouped(gpu, dt, nb, decomp):
+@pytest.mark.parametrize("dt,nb,decomp,dims,L", [
+    (np.float32, 32, 0, [65, 33, 65], 5), (np.float32, 32, 1, [65, 33, 65], 5), (np.float64, 52, 0, [65, 33, 65], 5),
+    (np.float32, 30, 0, [129, 65], 5), (np.float64, 40, 1, [1025], 6),
+])
+def test_plan_perbit_retrieve_equals_grouped(gpu, dt, nb, decomp, dims, L):
     """PerBit and GroupedBP keep the same magnitude bits and signs: a
     retrieval of the same planes per level is equal value for value. (Where
     no 1 bit is retrieved, GroupedBP's block sign can make a coefficient
     -0.0 and PerBit's +0.0, as in ProDM's decoders, so zeros may differ in
     sign.)"""
-    dims, L = [65, 33, 65], 5
     x, pp = _plan_case(dt, dims, L, nb, capi.PERBIT, decomp, 11)
     _, pg = _plan_case(dt, dims, L, nb, capi.GROUPED, decomp, 11)
     for k in ([nb] * (L + 1), [min(nb, 3 + 4 * l) for l in range(L + 1)], [0] + [nb // 2] * L):
