@@ -837,95 +837,67 @@ __device__ inline typename Traits<T>::Fixed pb_mag(T x, int shift, int compat) {
     return (F)(x < 0 ? (uint64_t)0 - (uint64_t)q : (uint64_t)q);
 }
 
-template <class T> struct PbRun;  // a lane's run: up to 2 B bits
-template <> struct PbRun<float> { using R = uint64_t; };
-template <> struct PbRun<double> { using R = unsigned __int128; };
-
 // wave-wide exclusive prefix sum and total (DPP reductions of the device library)
 extern "C" __device__ uint32_t __ockl_wfscan_add_u32(uint32_t, bool);
 extern "C" __device__ uint32_t __ockl_wfred_add_u32(uint32_t);
 
-// the lane's magnitudes as plane words: D[k] bit j = bit k of element j
-template <class T> struct PbPlanes;
-template <> struct PbPlanes<float> {
-    uint32_t m[32];
-    uint32_t sg;
-    __device__ void load(const float *row, int shift, int compat) {
-        sg = 0;
-#pragma unroll
-        for (int j = 0; j < 32; j++) {
-            m[j] = pb_mag<float>(row[j], shift, compat);
-            sg |= (uint32_t)(row[j] < 0) << j;
-        }
-        transpose32(m);
-    }
-    template <int K> __device__ uint64_t word() const { return m[K]; }
-    // plane words into the lane's own staging row (w[k] = word k)
-    __device__ void store(float *row) const {
-        uint32_t *w = reinterpret_cast<uint32_t *>(row);
-#pragma unroll
-        for (int k = 0; k < 32; k++) w[k] = m[k];
-    }
-};
-template <> struct PbPlanes<double> {
-    uint32_t lo[64], hi[64];
-    uint64_t sg;
-    __device__ void load(const double *row, int shift, int compat) {
-        sg = 0;
-#pragma unroll
-        for (int j = 0; j < 64; j++) {
-            const uint64_t q = pb_mag<double>(row[j], shift, compat);
-            lo[j] = (uint32_t)q;
-            hi[j] = (uint32_t)(q >> 32);
-            sg |= (uint64_t)(row[j] < 0) << j;
-        }
-        transpose64(lo, hi);
-    }
-    template <int K> __device__ uint64_t word() const { return word64_get<K>(lo, hi); }
-    __device__ void store(double *row) const {
-        uint64_t *w = reinterpret_cast<uint64_t *>(row);
-        static_for<0, 64>([&](auto kc) {
-            constexpr int k = decltype(kc)::value;
-            w[k] = word64_get<k>(lo, hi);
-        });
-    }
-};
+// Every lane takes 32 consecutive elements (f32 and f64), a wave 2048, a
+// workgroup of 4 waves 8192; a lane's run in a stream is at most 64 bits.
+constexpr int kPbTPB = 256, kPbB = 32;
 
-// f32: the lane's 32 elements straight from global memory (8 x 16 B when
-// the row is whole and aligned) and its plane words kept in registers, so
-// the workgroup needs no staging LDS.
-struct PbLane32 {
-    uint32_t m[32];
+// The lane's 32 elements straight from global memory (16-byte loads when
+// the row is whole and aligned) as plane words kept in registers: w[k] bit
+// j = bit k of element j's magnitude (k < 32 for f32, < 64 for f64); sg
+// bit j = element j < 0.
+template <class T> struct PbLane {
+    static constexpr int kWords = 8 * sizeof(T);
+    uint32_t w[kWords];
     uint32_t sg;
-    __device__ void load(const float *__restrict__ x, uint64_t e, uint32_t n, int shift, int compat, bool vec) {
-        float v[32];
-        if (vec && e + 32 <= n) {
-            const float4 *q = reinterpret_cast<const float4 *>(x + e);
+    __device__ __forceinline__ void load(const T *__restrict__ x, uint64_t e, uint32_t n, int shift, int compat, bool vec) {
+        T v[kPbB];
+        if (vec && e + kPbB <= n) {
+            if constexpr (sizeof(T) == 4) {
+                const float4 *q = reinterpret_cast<const float4 *>(x + e);
 #pragma unroll
-            for (int i = 0; i < 8; i++) {
-                const float4 t = q[i];
-                v[4 * i] = t.x; v[4 * i + 1] = t.y; v[4 * i + 2] = t.z; v[4 * i + 3] = t.w;
+                for (int i = 0; i < kPbB / 4; i++) {
+                    const float4 t = q[i];
+                    v[4 * i] = t.x; v[4 * i + 1] = t.y; v[4 * i + 2] = t.z; v[4 * i + 3] = t.w;
+                }
+            } else {
+                const double2 *q = reinterpret_cast<const double2 *>(x + e);
+#pragma unroll
+                for (int i = 0; i < kPbB / 2; i++) {
+                    const double2 t = q[i];
+                    v[2 * i] = t.x; v[2 * i + 1] = t.y;
+                }
             }
         } else {
 #pragma unroll
-            for (int j = 0; j < 32; j++) v[j] = e + j < n ? x[e + j] : 0.f;
+            for (int j = 0; j < kPbB; j++) v[j] = e + j < n ? x[e + j] : (T)0;
         }
         sg = 0;
 #pragma unroll
-        for (int j = 0; j < 32; j++) {
-            m[j] = pb_mag<float>(v[j], shift, compat);
+        for (int j = 0; j < kPbB; j++) {
+            const uint64_t q = pb_mag<T>(v[j], shift, compat);
+            w[j] = (uint32_t)q;
+            if constexpr (sizeof(T) == 8) w[32 + j] = (uint32_t)(q >> 32);
             sg |= (uint32_t)(v[j] < 0) << j;
         }
-        transpose32(m);
+        transpose32_at<0>(w);
+        if constexpr (sizeof(T) == 8) transpose32_at<32>(w);
     }
 };
-template <class Fn> __device__ __forceinline__ void pb_for_planes(const PbLane32 &P, int nb, Fn &&f) {
+// f(p, D, F) for the planes p = 0..nb-1 in order: D = the plane word (bit
+// nb - 1 - p of the lane's elements), F = the elements whose first 1 bit is
+// in plane p. Unrolled, so that the words stay in registers.
+template <class T, class Fn> __device__ __forceinline__ void pb_for_planes(const PbLane<T> &P, int nb, Fn &&f) {
+    constexpr int W = PbLane<T>::kWords;
     uint64_t H = 0;
-    static_for<0, 32>([&](auto kc) {  // unrolled: the words stay in registers
-        constexpr int k = 31 - decltype(kc)::value;
+    static_for<0, W>([&](auto kc) __attribute__((always_inline)) {
+        constexpr int k = W - 1 - decltype(kc)::value;
         const int p = nb - 1 - k;
         if (p >= 0) {
-            const uint64_t D = P.m[k];
+            const uint64_t D = P.w[k];
             const uint64_t F = D & ~H;
             H |= D;
             f(p, D, F);
@@ -933,29 +905,13 @@ template <class Fn> __device__ __forceinline__ void pb_for_planes(const PbLane32
     });
 }
 
-// The lane's plane words, moved to its own staging row after the
-// transpose: f(p, D, F) for the planes p = 0..nb-1 in order (D = the plane
-// word, F = the elements whose first 1 bit is in plane p). A loop over LDS
-// words rather than unrolled registers keeps the kernels small.
-template <class T> using PbWord = typename std::conditional<sizeof(T) == 4, uint32_t, uint64_t>::type;
-template <class T, class Fn> __device__ __forceinline__ void pb_for_planes(const PbWord<T> *w, int nb, Fn &&f) {
-    uint64_t H = 0;
-    for (int p = 0; p < nb; p++) {
-        const uint64_t D = w[nb - 1 - p];  // bit nb - 1 - p of the lane's elements
-        const uint64_t F = D & ~H;
-        H |= D;
-        f(p, D, F);
-    }
-}
-
-// Runs are cut per wave (64 lanes x B consecutive elements): the waves of
-// a workgroup share only the coalesced staging of the input.
-template <class T> static uint32_t pb_nwg(uint32_t n) { return div_up(n, (uint64_t)BlkStage<T>::kTPB * BlkStage<T>::kB); }
-template <class T> static uint32_t pb_nwaves(uint32_t n) { return pb_nwg<T>(n) * (BlkStage<T>::kTPB / 64); }
+// Runs are cut per wave (64 lanes x 32 consecutive elements).
+template <class T> static uint32_t pb_nwg(uint32_t n) { return div_up(n, (uint64_t)kPbTPB * kPbB); }
+template <class T> static uint32_t pb_nwaves(uint32_t n) { return pb_nwg<T>(n) * (kPbTPB / 64); }
 
 // valid elements of wave gw
 template <class T> __device__ inline uint32_t pb_wave_valid(uint32_t n, uint32_t gw) {
-    constexpr uint64_t E = 64ull * BlkStage<T>::kB;
+    constexpr uint64_t E = 64ull * kPbB;
     const uint64_t e = (uint64_t)gw * E;
     return e < n ? (uint32_t)min(E, (uint64_t)n - e) : 0u;
 }
@@ -963,34 +919,20 @@ template <class T> __device__ inline uint32_t pb_wave_valid(uint32_t n, uint32_t
 // Pass 1: bits[p][gw] = bits of wave gw's run in plane p: its valid
 // elements plus those whose first 1 bit is in plane p.
 template <class T>
-__global__ __launch_bounds__(BlkStage<T>::kTPB) void k_pb_count(const T *__restrict__ x, uint32_t n, int32_t exp,
-                                                             const KeyOf<T> *key, int nb, int compat,
-                                                             uint32_t *__restrict__ bits, uint32_t nwv) {
-    constexpr int TPB = BlkStage<T>::kTPB, B = BlkStage<T>::kB, PI = BlkStage<T>::kPitch, NW = TPB / 64;
+__global__ __launch_bounds__(kPbTPB) void k_pb_count(const T *__restrict__ x, uint32_t n, int32_t exp,
+                                                  const KeyOf<T> *key, int nb, int compat,
+                                                  uint32_t *__restrict__ bits, uint32_t nwv) {
+    constexpr int NW = kPbTPB / 64;
     const int lane = (int)lane_id();
     const uint32_t gw = blockIdx.x * NW + threadIdx.x / 64;
-    const int shift = nb - level_exp<T>(exp, key);
+    PbLane<T> P;
+    P.load(x, ((uint64_t)blockIdx.x * kPbTPB + threadIdx.x) * kPbB, n, nb - level_exp<T>(exp, key), compat,
+           ((uintptr_t)x & 15) == 0);
     uint32_t mine = 0;  // lane p: plane p's first-1 flags of the wave
-    auto count = [&](int p, uint64_t, uint64_t F) {
+    pb_for_planes(P, nb, [&](int p, uint64_t, uint64_t F) {
         const uint32_t s = __ockl_wfred_add_u32((uint32_t)__popcll(F));
         if (lane == p) mine = s;
-    };
-    if constexpr (sizeof(T) == 4) {
-        PbLane32 P;
-        P.load(x, ((uint64_t)blockIdx.x * TPB + threadIdx.x) * B, n, shift, compat, ((uintptr_t)x & 15) == 0);
-        pb_for_planes(P, nb, count);
-    } else {
-        __shared__ T sx[TPB * PI];
-        stage_in<T>(x, n, (uint64_t)blockIdx.x * TPB * B, sx);
-        __syncthreads();
-        T *row = sx + threadIdx.x * PI;
-        {
-            PbPlanes<T> P;
-            P.load(row, shift, compat);
-            P.store(row);
-        }
-        pb_for_planes<T>(reinterpret_cast<const PbWord<T> *>(row), nb, count);
-    }
+    });
     if (lane < nb) bits[(uint64_t)lane * nwv + gw] = pb_wave_valid<T>(n, gw) + mine;
 }
 
@@ -1066,13 +1008,12 @@ __global__ __launch_bounds__(kPbScanT) void k_pb_scan_off(const uint32_t *__rest
 // wave assembles its run in its own LDS words (in wave order, no
 // barriers) and stores them.
 template <class T>
-__global__ __launch_bounds__(BlkStage<T>::kTPB) void k_pb_write(const T *__restrict__ x, uint32_t n, int32_t exp,
-                                                             const KeyOf<T> *key, int nb, int compat,
-                                                             const uint64_t *__restrict__ off, uint32_t nwv,
-                                                             uint64_t *__restrict__ streams, uint64_t stride,
-                                                             uint64_t *__restrict__ side) {
-    using R = typename PbRun<T>::R;
-    constexpr int TPB = BlkStage<T>::kTPB, B = BlkStage<T>::kB, PI = BlkStage<T>::kPitch, NW = TPB / 64;
+__global__ __launch_bounds__(kPbTPB) void k_pb_write(const T *__restrict__ x, uint32_t n, int32_t exp,
+                                                  const KeyOf<T> *key, int nb, int compat,
+                                                  const uint64_t *__restrict__ off, uint32_t nwv,
+                                                  uint64_t *__restrict__ streams, uint64_t stride,
+                                                  uint64_t *__restrict__ side) {
+    constexpr int B = kPbB, NW = kPbTPB / 64;
     constexpr int kRunWords = (2 * 64 * B + 63) / 64 + 2;  // a wave's run plus its start offset
     __shared__ unsigned long long run[NW][kRunWords];
     const int lane = (int)lane_id();
@@ -1080,11 +1021,13 @@ __global__ __launch_bounds__(BlkStage<T>::kTPB) void k_pb_write(const T *__restr
     unsigned long long *buf = run[wv];
     for (int i = lane; i < kRunWords; i += 64) buf[i] = 0;
     const uint64_t my_off = lane < nb ? off[(uint64_t)lane * nwv + gw] : 0;  // lane p: plane p's offset
-    const int shift = nb - level_exp<T>(exp, key);
     const uint32_t nv = pb_wave_valid<T>(n, gw);
     const uint32_t below = min((uint32_t)lane * B, nv);  // valid elements of the lanes below
-    uint64_t S;  // sign bits of the lane's elements
-    auto write = [&](int p, uint64_t D, uint64_t F) {
+    PbLane<T> P;
+    P.load(x, ((uint64_t)blockIdx.x * kPbTPB + threadIdx.x) * B, n, nb - level_exp<T>(exp, key), compat,
+           ((uintptr_t)x & 15) == 0);
+    const uint64_t S = P.sg;  // sign bits of the lane's elements
+    pb_for_planes(P, nb, [&](int p, uint64_t D, uint64_t F) __attribute__((always_inline)) {
         const uint64_t o = ((uint64_t)readlane_u32((uint32_t)(my_off >> 32), p) << 32) |
                            readlane_u32((uint32_t)my_off, p);
         const uint32_t c = (uint32_t)__popcll(F);
@@ -1092,28 +1035,18 @@ __global__ __launch_bounds__(BlkStage<T>::kTPB) void k_pb_write(const T *__restr
         const uint32_t flags = readlane_u32(pre + c, 63);  // the wave's total
         const uint32_t q = (uint32_t)(o & 63) + below + pre;
         // insert each flagged element's sign after its bit, highest first
-        R r = (R)D;
+        uint64_t r = D;
         uint64_t ff = F;
         while (ff) {
             const int j = 63 - __clzll(ff);
             ff ^= 1ull << j;
-            const R low = ((R)2 << j) - 1;
-            r = (r & low) | ((R)((S >> j) & 1u) << (j + 1)) | ((r & ~low) << 1);
+            const uint64_t low = (2ull << j) - 1;
+            r = (r & low) | (((S >> j) & 1u) << (j + 1)) | ((r & ~low) << 1);
         }
         const uint32_t wi = q >> 6, sh = q & 63;
-        if constexpr (sizeof(T) == 4) {
-            const uint64_t w0 = r << sh, w1 = sh ? r >> (64 - sh) : 0;
-            if (w0) atomicOr(&buf[wi], w0);
-            if (w1) atomicOr(&buf[wi + 1], w1);
-        } else {
-            const uint64_t rl = (uint64_t)r, rh = (uint64_t)(r >> 64);
-            const uint64_t w0 = rl << sh;
-            const uint64_t w1 = sh ? (rh << sh) | (rl >> (64 - sh)) : rh;
-            const uint64_t w2 = sh ? rh >> (64 - sh) : 0;
-            if (w0) atomicOr(&buf[wi], w0);
-            if (w1) atomicOr(&buf[wi + 1], w1);
-            if (w2) atomicOr(&buf[wi + 2], w2);
-        }
+        const uint64_t w0 = r << sh, w1 = sh ? r >> (64 - sh) : 0;
+        if (w0) atomicOr(&buf[wi], w0);
+        if (w1) atomicOr(&buf[wi + 1], w1);
         // whole words of this run are stored; a first word shared with the
         // run below (head) and a last word not wholly this run's (tail) go to
         // side[p][gw] for k_pb_fixup
@@ -1129,29 +1062,11 @@ __global__ __launch_bounds__(BlkStage<T>::kTPB) void k_pb_write(const T *__restr
             else if (w == nwords - 1 && (end & 63)) sd[1] = v;
             else st[w] = v;
         }
-    };
-    if constexpr (sizeof(T) == 4) {
-        PbLane32 P;
-        P.load(x, ((uint64_t)blockIdx.x * TPB + threadIdx.x) * B, n, shift, compat, ((uintptr_t)x & 15) == 0);
-        S = P.sg;
-        pb_for_planes(P, nb, write);
-    } else {
-        __shared__ T sx[TPB * PI];
-        stage_in<T>(x, n, (uint64_t)blockIdx.x * TPB * B, sx);
-        __syncthreads();
-        T *row = sx + threadIdx.x * PI;
-        {
-            PbPlanes<T> P;
-            P.load(row, shift, compat);
-            P.store(row);
-            S = P.sg;
-        }
-        pb_for_planes<T>(reinterpret_cast<const PbWord<T> *>(row), nb, write);
-    }
+    });
 }
 
 // Pass 4: the words at run boundaries. Only the last non-empty run can be
-// shorter than a word (the others hold 64 B elements), so a word is shared
+// shorter than a word (the others hold 2048 elements), so a word is shared
 // by at most two runs: run g's head word is its head ORed with run g - 1's
 // tail; the last run's tail word is its own.
 __global__ __launch_bounds__(256) void k_pb_fixup(const uint32_t *__restrict__ bits, const uint64_t *__restrict__ off,
@@ -1190,16 +1105,16 @@ int launch_pb_encode(const T *x, uint32_t n, int32_t exp, const KeyOf<T> *key, i
     uint64_t *side = off + (size_t)nwv * nb;
     uint64_t *sums = side + 2 * (size_t)nwv * nb;
     uint32_t *bits = reinterpret_cast<uint32_t *>(sums + (size_t)nblk * nb);
-    hipLaunchKernelGGL(k_pb_count<T>, dim3(nwg), dim3(BlkStage<T>::kTPB), 0, s, x, n, exp, key, nb, compat, bits, nwv);
+    hipLaunchKernelGGL(k_pb_count<T>, dim3(nwg), dim3(kPbTPB), 0, s, x, n, exp, key, nb, compat, bits, nwv);
     PRODM_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_pb_scan_sum, dim3(nblk, nb), dim3(kPbScanT), 0, s, bits, nwv, sums);
     PRODM_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_pb_scan_off, dim3(nblk, nb), dim3(kPbScanT), 0, s, bits, nwv, sums, off, sizes);
     PRODM_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_pb_write<T>, dim3(nwg), dim3(BlkStage<T>::kTPB), 0, s, x, n, exp, key, nb, compat, off, nwv,
+    hipLaunchKernelGGL(k_pb_write<T>, dim3(nwg), dim3(kPbTPB), 0, s, x, n, exp, key, nb, compat, off, nwv,
                        streams, stride, side);
     PRODM_CHECK_LAUNCH();
-    const uint32_t glast = (n - 1) / (64u * BlkStage<T>::kB);  // the last wave with elements
+    const uint32_t glast = (n - 1) / (64u * kPbB);  // the last wave with elements
     hipLaunchKernelGGL(k_pb_fixup, dim3(div_up(glast + 1, 256), nb), dim3(256), 0, s, bits, off, side, nwv, glast,
                        streams, stride);
     PRODM_CHECK_LAUNCH();
