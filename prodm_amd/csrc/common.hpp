@@ -152,6 +152,32 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) /
 // kernel runs (wave-uniform loads from it are scalar loads).
 #define PRODM_CONST __attribute__((address_space(4)))
 
+// Raw buffer accesses: a wave-uniform base in a buffer descriptor plus a
+// 32-bit lane byte offset. A lane whose offset is kOffNone lies outside the
+// descriptor's range, so the hardware returns 0 for its load and drops its
+// store: masked lanes need no exec-mask branches. A descriptor made with
+// on = false has range 0 and masks the whole wave. (Descriptor word 3 =
+// 32-bit data format, the gfx9 setting for untyped raw accesses.)
+constexpr uint32_t kOffNone = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, bool on) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, on ? 0x7FFFFFFF : 0, 0x00020000);
+}
+template <class T> __device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+    }
+}
+template <class T> __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, T v) {
+    if constexpr (sizeof(T) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)off, 0, 0);
+    } else {
+        using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, (int)off, 0, 0);
+    }
+}
+
 // A parameter struct at a wave-uniform address, copied into scalar
 // registers word by word (every lane reads the same words).
 template <class S> __device__ __forceinline__ S uniform_load(const S *p) {

@@ -163,7 +163,6 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt) {
     // may come from memory (grouped launches), where the compiler cannot
     // infer the address space and would emit flat_* operations
     using GT = PRODM_GLOBAL T;
-    using GChar = PRODM_GLOBAL char;
     constexpr bool kDec = MODE == kDecompose;
     constexpr int kPE = kDec ? kPL : 1;  // input planes staged only by decompose
     __shared__ T E[kPE], O[kPE], Wb[kPL], Wb2[kPE];
@@ -197,7 +196,7 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt) {
         bits |= (uint32_t)(valid | (o1 << 1) | (o2 << 2) | ((own1 && own2) << 3)) << (4 * m);
         qb[m] = valid ? (o1 ? r1 - 1 : r1) * kN2 + (o2 ? r2 - 1 : r2) : 0;
         const uint32_t u1 = valid ? (uint32_t)i1 : 0, u2 = valid ? (uint32_t)i2 : 0;
-        in_off[m] = (uint32_t)(u1 * (uint32_t)P.is[1] + u2) * (uint32_t)sizeof(T);
+        in_off[m] = valid ? (uint32_t)(u1 * (uint32_t)P.is[1] + u2) * (uint32_t)sizeof(T) : kOffNone;
         const uint32_t p1 = rpos(u1, c1), p2 = rpos(u2, c2);
         uint32_t a, bb;
         if (ILV) {
@@ -221,23 +220,29 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt) {
         return (uint64_t)rpos((uint32_t)i0, c0) * P.cs[0];
     };
 
+    // fine plane i0 into registers; slots outside the box read 0
     auto fetch = [&](T (&reg)[kPER], int i0) __attribute__((always_inline)) {
         const bool ok = i0 >= 0 && i0 < (int)n0;
-        const bool odd0 = act0 && (i0 & 1);
-        const GChar *pb = MODE == kRecompose ? (const GChar *)((const GT *)(ILV ? P.coef_ilv : P.coef_full) + coef_base(i0))
-                                             : (const GChar *)((const GT *)P.in + (uint64_t)i0 * P.is[0]);
+        if constexpr (MODE != kRecompose) {
+            // the input plane (invalid slots carry kOffNone)
+            const auto r = buf_rsrc((const void *)((const GT *)P.in + (uint64_t)(ok ? i0 : 0) * P.is[0]), ok);
 #pragma unroll
-        for (int m = 0; m < kPER; m++) {
-            const uint32_t b = bits >> (4 * m);
-            T v = (T)0;
-            if (ok && (b & 1)) {
-                if (MODE == kRecompose) {
-                    if (odd0 || (b & 6)) v = *(const GT *)(pb + (odd0 ? offB[m] : offA[m]));
-                } else {
-                    v = *(const GT *)(pb + in_off[m]);
-                }
+            for (int m = 0; m < kPER; m++) reg[m] = buf_load<T>(r, in_off[m]);
+        } else {
+            // W from the coefficients: odd planes hold every node, even
+            // planes all but the coarse ones (W = 0 there). Predicated
+            // global loads: measured faster here than buffer loads with
+            // masked offsets.
+            const bool odd0 = act0 && (i0 & 1);
+            const char *pb = (const char *)((const GT *)(ILV ? P.coef_ilv : P.coef_full) + coef_base(i0));
+#pragma unroll
+            for (int m = 0; m < kPER; m++) {
+                const uint32_t b = bits >> (4 * m);
+                T v = (T)0;
+                if (ok && (b & 1) && (odd0 || (b & 6)))
+                    v = *(const GT *)((const PRODM_GLOBAL char *)pb + (odd0 ? offB[m] : offA[m]));
+                reg[m] = v;
             }
-            reg[m] = v;
         }
     };
     auto put = [&](T *dst, const T (&reg)[kPER]) __attribute__((always_inline)) {
@@ -285,8 +290,9 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt) {
     // values into u_c and coefficients into the level buffer
     auto even_plane = [&](int i0, const T *Ev, T *Wout, T (&s)[kPER]) __attribute__((always_inline)) {
         const bool own0 = own_plane(i0);
-        GChar *cp = (GChar *)((GT *)(ILV ? P.coef_ilv : P.coef_full) + coef_base(i0));
-        GChar *up = (GChar *)((GT *)P.uc + (uint64_t)(i0 >> 1) * c1 * c2);
+        // planes of the halo store nothing (range 0)
+        const auto rc = buf_rsrc((const void *)((GT *)(ILV ? P.coef_ilv : P.coef_full) + coef_base(i0)), own0);
+        const auto ru = buf_rsrc((const void *)((GT *)P.uc + (uint64_t)(i0 >> 1) * c1 * c2), own0);
 #pragma unroll
         for (int m = 0; m < kPER; m++) {
             const int idx = tid + kLoadThreads * m;
@@ -300,20 +306,20 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt) {
             const bool coarse = !(b & 6);
             const T w = ((b & 1) && !coarse) ? cv - S * (T)0.25 : (T)0;
             Wout[idx] = w;
-            // one predicated store: the coarse value into u_c or the
-            // coefficient into the level buffer
-            const bool st = (b & 1) && (b & 8) && own0;
+            // the coarse value into u_c or the coefficient into the level
+            // buffer; slots not stored by this tile take kOffNone
+            const bool st = (b & 1) && (b & 8);
             const uint32_t oa = offA[m];
-            GChar *dst = coarse ? up + oa : cp + oa;
-            if (st) *(GT *)dst = coarse ? cv : w;
-            keep_max(st && !coarse, w);
+            buf_store<T>(rc, st && !coarse ? oa : kOffNone, w);
+            buf_store<T>(ru, st && coarse ? oa : kOffNone, cv);
+            keep_max(st && own0 && !coarse, w);
         }
     };
     // odd plane i0 staged in Ov, between even planes with sums sa and sb
     auto odd_plane = [&](int i0, const T *Ov, T *Wout, const T (&sa)[kPER], const T (&sb)[kPER])
         __attribute__((always_inline)) {
         const bool own0 = own_plane(i0);
-        GChar *cp = (GChar *)((GT *)(ILV ? P.coef_ilv : P.coef_full) + coef_base(i0));
+        const auto rc = buf_rsrc((const void *)((GT *)(ILV ? P.coef_ilv : P.coef_full) + coef_base(i0)), own0);
 #pragma unroll
         for (int m = 0; m < kPER; m++) {
             const int idx = tid + kLoadThreads * m;
@@ -322,9 +328,9 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt) {
             const T I = (sa[m] + sb[m]) * (T)0.125;
             const T w = (b & 1) ? Ov[idx] - I : (T)0;
             Wout[idx] = w;
-            const bool st = (b & 1) && (b & 8) && own0;
-            if (st) *(GT *)(cp + offB[m]) = w;
-            keep_max(st, w);
+            const bool st = (b & 1) && (b & 8);
+            buf_store<T>(rc, st ? offB[m] : kOffNone, w);
+            keep_max(st && own0, w);
         }
     };
     // load stencil along dim 2 from Wb into G2[r1][jj2]; this thread's jj2
