@@ -528,18 +528,33 @@ __device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint
     const uint32_t lane = lane_id(), wv = threadIdx.x / 64;
     const uint64_t first = line0 + (uint64_t)wv * NL;
     if (first >= nlines) return;
-    const PRODM_GLOBAL double *inv = (const PRODM_GLOBAL double *)S.inv;
     const uint32_t seg = (m + 63) / 64;
     const uint32_t b = lane * seg;
     const uint32_t e = min(m, b + seg);
-    bool ok[NL];
+    // Slot i of the lane is node b + i when in[i]. The factors are loaded
+    // once; slots past the segment compute on zeros and are discarded by
+    // selects (no exec-mask branches), and their loads and stores carry
+    // kOffNone. Lines past nlines get zero-range descriptors.
+    bool in[SEG];
+    double iv[SEG];
+    const auto rinv = buf_rsrc((const void *)S.inv, true);
+    auto off = [&](int i) __attribute__((always_inline)) -> uint32_t { return in[i] ? (b + i) * 8u : kOffNone; };
+#pragma unroll
+    for (int i = 0; i < SEG; i++) {
+        const uint32_t k = b + i;
+        in[i] = k < e;
+        const double t = buf_load<double>(rinv, k < S.nfix && in[i] ? k * 8u : kOffNone);
+        iv[i] = k < S.nfix ? t : (k == m - 1 ? S.last : S.lim);
+    }
+    auto fac = [&](int i) __attribute__((always_inline)) -> double { return iv[i]; };
+    __amdgpu_buffer_rsrc_t rl[NL];
     double v[NL][SEG];
 #pragma unroll
     for (int p = 0; p < NL; p++) {
-        ok[p] = first + p < nlines;
-        const PRODM_GLOBAL double *x = (const PRODM_GLOBAL double *)S.f + (first + p) * S.fp;
+        const bool ok = first + p < nlines;
+        rl[p] = buf_rsrc((const void *)((const PRODM_GLOBAL double *)S.f + (ok ? first + p : 0) * S.fp), ok);
 #pragma unroll
-        for (int i = 0; i < SEG; i++) v[p][i] = (ok[p] && b + i < e) ? x[b + i] : 0.0;
+        for (int i = 0; i < SEG; i++) v[p][i] = buf_load<double>(rl[p], off(i));
     }
     // forward: y_i = (f_i - y_{i-1}) * inv_i  == affine map y_{i-1} -> y_i
     double A[NL], Bm[NL];
@@ -547,13 +562,12 @@ __device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint
     for (int p = 0; p < NL; p++) { A[p] = 0.0; Bm[p] = 1.0; }
 #pragma unroll
     for (int i = 0; i < SEG; i++) {
-        if (b + i < e) {
-            const double iv = thomas_factor(S, inv, m, b + i);
+        const double f = fac(i);
 #pragma unroll
-            for (int p = 0; p < NL; p++) {
-                A[p] = (v[p][i] - A[p]) * iv;
-                Bm[p] = -Bm[p] * iv;
-            }
+        for (int p = 0; p < NL; p++) {
+            const double nA = (v[p][i] - A[p]) * f, nB = -Bm[p] * f;
+            A[p] = in[i] ? nA : A[p];
+            Bm[p] = in[i] ? nB : Bm[p];
         }
     }
     for (int o = 1; o < 64; o <<= 1) {
@@ -571,13 +585,12 @@ __device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint
     }
 #pragma unroll
     for (int i = 0; i < SEG; i++) {
-        if (b + i < e) {
-            const double iv = thomas_factor(S, inv, m, b + i);
+        const double f = fac(i);
 #pragma unroll
-            for (int p = 0; p < NL; p++) {
-                y[p] = (v[p][i] - y[p]) * iv;
-                v[p][i] = y[p];
-            }
+        for (int p = 0; p < NL; p++) {
+            const double ny = (v[p][i] - y[p]) * f;
+            y[p] = in[i] ? ny : y[p];
+            v[p][i] = ny;
         }
     }
     // backward: z_i = y_i - cp_i z_{i+1} (z_{m-1} = y_{m-1})
@@ -586,13 +599,12 @@ __device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint
     for (int p = 0; p < NL; p++) { Cc[p] = 0.0; Dm[p] = 1.0; }
 #pragma unroll
     for (int i = SEG - 1; i >= 0; i--) {
-        if (b + i < e) {
-            const double c = (b + i == m - 1) ? 0.0 : thomas_factor(S, inv, m, b + i);
+        const double c = (b + i == m - 1) ? 0.0 : fac(i);
 #pragma unroll
-            for (int p = 0; p < NL; p++) {
-                Cc[p] = v[p][i] - c * Cc[p];
-                Dm[p] = -c * Dm[p];
-            }
+        for (int p = 0; p < NL; p++) {
+            const double nC = v[p][i] - c * Cc[p], nD = -c * Dm[p];
+            Cc[p] = in[i] ? nC : Cc[p];
+            Dm[p] = in[i] ? nD : Dm[p];
         }
     }
     for (int o = 1; o < 64; o <<= 1) {
@@ -610,22 +622,28 @@ __device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint
     }
 #pragma unroll
     for (int i = SEG - 1; i >= 0; i--) {
-        if (b + i < e) {
-            const double c = (b + i == m - 1) ? 0.0 : thomas_factor(S, inv, m, b + i);
+        const double c = (b + i == m - 1) ? 0.0 : fac(i);
 #pragma unroll
-            for (int p = 0; p < NL; p++) {
-                z[p] = v[p][i] - c * z[p];
-                v[p][i] = z[p];
-            }
+        for (int p = 0; p < NL; p++) {
+            const double nz = v[p][i] - c * z[p];
+            z[p] = in[i] ? nz : z[p];
+            v[p][i] = nz;
         }
+    }
+    if (S.epi == kEpiNone) {
+#pragma unroll
+        for (int p = 0; p < NL; p++)
+#pragma unroll
+            for (int i = 0; i < SEG; i++) buf_store<double>(rl[p], off(i), v[p][i]);
+        return;
     }
 #pragma unroll
     for (int p = 0; p < NL; p++) {
-        if (!ok[p]) continue;
+        if (first + p >= nlines) continue;
         const uint64_t line = first + p;
 #pragma unroll
         for (int i = 0; i < SEG; i++)
-            if (b + i < e) epilogue(S, line * S.fp + b + i, line * m + b + i, v[p][i]);
+            if (in[i]) epilogue(S, line * S.fp + b + i, line * m + b + i, v[p][i]);
     }
 }
 template <class T, int SEG>
@@ -984,6 +1002,8 @@ void set_factors(SolveParams<T> &S, const double *tab_dev, const Factors &F, int
     }
 }
 
+int seg_class_contig(uint32_t m);  // registers per lane of the contiguous solve (0: line too long)
+
 template <class T>
 int solve_all(const LevelGeom &g, int l, double *f, const double *tab_dev, const Factors &F, int epi, const T *uc_in,
               T *uc_out, hipStream_t s) {
@@ -1008,10 +1028,14 @@ int solve_all(const LevelGeom &g, int l, double *f, const double *tab_dev, const
             const uint64_t nlines = (uint64_t)c[0] * c[1];
             const uint32_t grid = div_up(nlines, 4 * kContigLPW);
             const uint32_t seg = (m + 63) / 64;
-            if (seg <= 4) hipLaunchKernelGGL((k_solve_contig<T, 4>), dim3(grid), dim3(256), 0, s, S, nlines);
-            else if (seg <= 8) hipLaunchKernelGGL((k_solve_contig<T, 8>), dim3(grid), dim3(256), 0, s, S, nlines);
-            else if (seg <= 16) hipLaunchKernelGGL((k_solve_contig<T, 16>), dim3(grid), dim3(256), 0, s, S, nlines);
-            else { set_error("line too long for the contiguous solve"); return kErrUnsupported; }
+            switch (seg_class_contig(m)) {
+            case 4: hipLaunchKernelGGL((k_solve_contig<T, 4>), dim3(grid), dim3(256), 0, s, S, nlines); break;
+            case 5: hipLaunchKernelGGL((k_solve_contig<T, 5>), dim3(grid), dim3(256), 0, s, S, nlines); break;
+            case 8: hipLaunchKernelGGL((k_solve_contig<T, 8>), dim3(grid), dim3(256), 0, s, S, nlines); break;
+            case 9: hipLaunchKernelGGL((k_solve_contig<T, 9>), dim3(grid), dim3(256), 0, s, S, nlines); break;
+            case 16: hipLaunchKernelGGL((k_solve_contig<T, 16>), dim3(grid), dim3(256), 0, s, S, nlines); break;
+            default: set_error("line too long for the contiguous solve"); return kErrUnsupported;
+            }
         } else {
             const uint64_t nlines = (uint64_t)(d == 1 ? c[0] : c[1]) * S.fp;
             // up to 8 waves split a line; each lane keeps its segment in registers
@@ -1296,7 +1320,7 @@ template <class E> void fill_sorted(GroupLaunch &L, std::vector<std::pair<uint32
 
 int seg_class_contig(uint32_t m) {
     const uint32_t seg = (m + 63) / 64;
-    return seg <= 4 ? 4 : seg <= 8 ? 8 : seg <= 16 ? 16 : 0;
+    return seg <= 4 ? 4 : seg == 5 ? 5 : seg <= 8 ? 8 : seg == 9 ? 9 : seg <= 16 ? 16 : 0;
 }
 int seg_class_strided(uint32_t seg) {
     return seg <= 8 ? 8 : seg <= 16 ? 16 : seg <= 24 ? 24 : seg <= 40 ? 40 : seg <= 72 ? 72 : 0;
@@ -1473,7 +1497,9 @@ template <class T> int group_run_t(const GroupLaunch &L, hipStream_t s) {
     case kGrpLoad: PRODM_GRP((k_load_grouped<T>), LoadParams<T>); break;
     case kGrpContig:
         if (L.variant == 4) PRODM_GRP((k_contig_grouped<T, 4>), SolveEntry<T>);
+        else if (L.variant == 5) PRODM_GRP((k_contig_grouped<T, 5>), SolveEntry<T>);
         else if (L.variant == 8) PRODM_GRP((k_contig_grouped<T, 8>), SolveEntry<T>);
+        else if (L.variant == 9) PRODM_GRP((k_contig_grouped<T, 9>), SolveEntry<T>);
         else if (L.variant == 16) PRODM_GRP((k_contig_grouped<T, 16>), SolveEntry<T>);
         else { set_error("grouped contiguous solve: bad variant"); return kErrArg; }
         break;
