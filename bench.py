@@ -376,8 +376,8 @@ def main():
         calibrate_copy()
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}): this build runs on a single GPU and "
-              "has no multi-GPU path.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}): the refactor/retrieve path runs on a "
+              "single GPU and has no multi-GPU path.", file=sys.stderr)
         sys.exit(2)
 
     import torch
