@@ -1045,8 +1045,8 @@ __global__ __launch_bounds__(kPbTPB) void k_pb_write(const T *__restrict__ x, ui
         }
         const uint32_t wi = q >> 6, sh = q & 63;
         const uint64_t w0 = r << sh, w1 = sh ? r >> (64 - sh) : 0;
-        if (w0) atomicOr(&buf[wi], w0);
-        if (w1) atomicOr(&buf[wi + 1], w1);
+        atomicOr(&buf[wi], w0);  // unconditional: OR-ing zero is harmless and needs no branch
+        atomicOr(&buf[wi + 1], w1);
         // whole words of this run are stored; a first word shared with the
         // run below (head) and a last word not wholly this run's (tail) go to
         // side[p][gw] for k_pb_fixup
@@ -1058,9 +1058,9 @@ __global__ __launch_bounds__(kPbTPB) void k_pb_write(const T *__restrict__ x, ui
             const uint64_t v = buf[w];
             buf[w] = 0;
             const bool head = w == 0 && (o & 63);
-            if (head) sd[0] = v;
-            else if (w == nwords - 1 && (end & 63)) sd[1] = v;
-            else st[w] = v;
+            const bool tail = !head && w == nwords - 1 && (end & 63);
+            // one store to the selected word (no branches)
+            *(head ? sd : tail ? sd + 1 : st + w) = v;
         }
     });
 }
