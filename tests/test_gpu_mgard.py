@@ -16,8 +16,9 @@ TOL = {np.float32: 1e-6, np.float64: 1e-14}
 GEOMS = [([129], 5), ([33, 65], 4), ([65, 33], 3), ([17, 33, 9], 2), ([65, 65, 65], 4), ([129, 65, 33], 4),
          ([3, 5, 3], 1), ([9, 9, 9], 3), ([1, 33, 33], 3), ([33, 1, 17], 2),
          # contiguous lines of 257, 401, 513 and 801 coarse nodes: 5, 7, 9 and
-         # 13 nodes per lane in the dim-2 solve
-         ([5, 513], 1), ([3, 801], 1), ([3, 1025], 1), ([3, 1601], 1)]
+         # 13 nodes per lane in the dim-2 solve; in 1-D that solve also
+         # updates the coarse values
+         ([5, 513], 1), ([3, 801], 1), ([3, 1025], 1), ([3, 1601], 1), ([1025], 2), ([1601], 1)]
 
 
 def rel(a, b):
