@@ -19,6 +19,8 @@ TD = {np.float32: torch.float32, np.float64: torch.float64}
 MIXED = [  # (dims, L): 2-D and 3-D fields of uneven shapes
     ([65, 65, 65], 5), ([33, 129], 4), ([129, 33, 65], 4), ([257, 257], 6), ([33, 33, 33], 4),
     ([65, 257], 5), ([129, 129, 33], 4), ([513, 65], 5), ([17, 17, 17], 3), ([65, 33, 129], 4),
+    # contiguous lines of 257 and 513 coarse nodes (5 and 9 per lane)
+    ([5, 513], 2), ([9, 1025], 3),
 ]
 
 
