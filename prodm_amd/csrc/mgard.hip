@@ -125,6 +125,17 @@ __device__ inline uint32_t xcd_remap(uint32_t bid, uint32_t nblocks) {
     return (bid % 8) * q + bid / 8;
 }
 
+// Tiles along dim 0: floor(c0 / march), the last one also taking the
+// remainder. Coarse extents are 2^k + 1, so rounding up would leave a last
+// tile of a single plane (257 = 16 x 16 + 1) that costs a whole tile's set-up;
+// here the last tile marches 17 planes instead. Which workgroup owns a coarse
+// plane does not change any result.
+__host__ __device__ inline uint32_t march_tiles0(uint32_t n0, uint32_t c0, int march) {
+    if (n0 == 1) return 1;
+    const uint32_t t = c0 / (uint32_t)march;
+    return t ? t : 1;
+}
+
 struct MarchTile {
     int js0, tj0, js1, tj1, js2, tj2;  // coarse tile
     int rs1, rs2;                      // fine region start (dims 1, 2)
@@ -143,7 +154,7 @@ __device__ inline MarchTile march_tile(uint32_t bt, uint32_t n0, uint32_t n1, ui
     t.tj1 = (n[1] == 1) ? 1 : min(kM1, (int)c[1] - t.js1);
     t.rs1 = (n[1] == 1) ? 0 : 2 * t.js1 - 2;
     t.js0 = (n[0] == 1) ? 0 : (int)a0 * kM0;
-    t.tj0 = (n[0] == 1) ? 1 : min(kM0, (int)c[0] - t.js0);
+    t.tj0 = (n[0] == 1) ? 1 : (a0 + 1 == march_tiles0(n[0], c[0], kM0) ? (int)c[0] - t.js0 : kM0);
     return t;
 }
 
@@ -151,7 +162,7 @@ __device__ inline MarchTile march_tile(uint32_t bt, uint32_t n0, uint32_t n1, ui
 template <class T> __host__ __device__ inline uint32_t load_tiles(const LoadParams<T> &P) {
     const uint32_t t2n = (P.n[2] == 1) ? 1 : (P.c[2] + kM2 - 1) / kM2;
     const uint32_t t1n = (P.n[1] == 1) ? 1 : (P.c[1] + kM1 - 1) / kM1;
-    const uint32_t t0n = (P.n[0] == 1) ? 1 : (P.c[0] + P.march - 1) / P.march;
+    const uint32_t t0n = march_tiles0(P.n[0], P.c[0], P.march);
     return t0n * t1n * t2n;
 }
 
@@ -1082,10 +1093,9 @@ int launch_load(LoadParams<T> P, bool ilv, hipStream_t s) {
     // the recompose sweep is fastest shortening below 2048.
     int march = 16;
     const uint64_t want = MODE == kDecompose ? 256 : 2048;
-    while (march > 2 && (uint64_t)t2n * t1n * div_up(P.c[0], march) < want) march /= 2;
+    while (march > 2 && (uint64_t)t2n * t1n * march_tiles0(P.n[0], P.c[0], march) < want) march /= 2;
     P.march = march;
-    const uint32_t t0n = (P.n[0] == 1) ? 1 : div_up(P.c[0], march);
-    const uint32_t grid = t2n * t1n * t0n;
+    const uint32_t grid = t2n * t1n * march_tiles0(P.n[0], P.c[0], march);
     if (ilv) hipLaunchKernelGGL((k_load<T, MODE, true>), dim3(grid), dim3(kLoadThreads), 0, s, P);
     else hipLaunchKernelGGL((k_load<T, MODE, false>), dim3(grid), dim3(kLoadThreads), 0, s, P);
     PRODM_CHECK_LAUNCH();
@@ -1334,7 +1344,7 @@ template <class T> int default_march(const LoadParams<T> &P) {
     const uint32_t t2n = (P.n[2] == 1) ? 1 : div_up(P.c[2], kM2);
     const uint32_t t1n = (P.n[1] == 1) ? 1 : div_up(P.c[1], kM1);
     int march = 16;
-    while (march > 2 && (uint64_t)t2n * t1n * div_up(P.c[0], march) < 2048) march /= 2;
+    while (march > 2 && (uint64_t)t2n * t1n * march_tiles0(P.n[0], P.c[0], march) < 2048) march /= 2;
     return march;
 }
 
