@@ -1081,8 +1081,23 @@ LoadParams<T> load_params(const LevelGeom &g, int l, const CoefSink<T> &sink) {
     return P;
 }
 
+// The decompose and adjoint sweeps address an input plane, a coefficient
+// plane and a coarse-value plane through buffer descriptors of range
+// 2^31 - 1 bytes (masked lanes use the offset 2^31), so each plane's byte
+// extent must stay below 2^31.
+template <class T> bool sweep_planes_fit(const LoadParams<T> &P) {
+    const uint64_t lim = 0x7FFFFFFFull;
+    const uint64_t in_plane = ((uint64_t)(P.n[1] - 1) * P.is[1] + P.n[2]) * sizeof(T);
+    const uint64_t coef_plane = (uint64_t)P.n[1] * std::max<uint64_t>(P.n[2], P.cs[1]) * sizeof(T);
+    return in_plane < lim && coef_plane < lim;
+}
+
 template <class T, int MODE>
 int launch_load(LoadParams<T> P, bool ilv, hipStream_t s) {
+    if (MODE != kRecompose && !sweep_planes_fit(P)) {
+        set_error("a fine plane of this level spans 2 GiB or more; the sweeps address planes with 31-bit offsets");
+        return kErrUnsupported;
+    }
     const uint32_t t2n = (P.n[2] == 1) ? 1 : div_up(P.c[2], kM2);
     const uint32_t t1n = (P.n[1] == 1) ? 1 : div_up(P.c[1], kM1);
     // march length: 16 coarse planes per workgroup, shorter when the level
@@ -1409,6 +1424,11 @@ int mgard_batched_launches(const std::vector<BatchFieldRef> &fields, const std::
             P.key = static_cast<KeyOf<T> *>(f.keys) + l;
             P.hb = f.hier;
             P.march = load.shape.a > 0 ? load.shape.a : default_march(P);
+            if (!sweep_planes_fit(P)) {
+                set_error("batched decompose: field " + std::to_string(i) +
+                          " has a fine plane of 2 GiB or more; the sweeps address planes with 31-bit offsets");
+                return kErrUnsupported;
+            }
             le.push_back({load_tiles(P), P});
             if (!f.hier) {
                 const uint32_t *c = g.ld[l - 1], *n = g.ld[l];
