@@ -13,8 +13,8 @@
 // Kernels per level:
 //   k_load    fine sweep, marching along dim 0 with a (dim1, dim2) tile in LDS:
 //             W, coefficient stores, u_c, max |W| and the 3-D load stencil.
-//   k_solve_contig   line solves along dim 2 (contiguous): one wave per line,
-//             segment per lane, affine-map scan across lanes.
+//   k_solve_contig   line solves along dim 2 (contiguous): two lines per wave,
+//             a segment of each per lane, affine-map scan across lanes.
 //   k_solve_strided  line solves along dims 1 / 0: lane per line (coalesced),
 //             waves split the line into segments, LDS scan across waves;
 //             the last solve applies u_c +/- z.
