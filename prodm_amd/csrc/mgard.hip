@@ -737,18 +737,15 @@ __device__ __forceinline__ void solve_strided_lines(const SolveParams<T> &S, uin
     __syncthreads();
     double z = 0.0;
     for (int w = (int)nw - 1; w > (int)wv; w--) z = sA[w][lane] + sB[w][lane] * z;
+    // each node's result is stored as soon as it is known (its epilogue
+    // loads do not wait for the end of the segment)
 #pragma unroll
     for (int i = SEG - 1; i >= 0; i--) {
         if (b + i < e) {
             const double c = (b + i == m - 1) ? 0.0 : thomas_factor(S, inv, m, b + i);
             z = v[i] - c * z;
-            v[i] = z;
+            if (act) epilogue(S, base + (uint64_t)(b + i) * stride, ubase + (uint64_t)(b + i) * ustride, z);
         }
-    }
-    if (act) {
-#pragma unroll
-        for (int i = 0; i < SEG; i++)
-            if (b + i < e) epilogue(S, base + (uint64_t)(b + i) * stride, ubase + (uint64_t)(b + i) * ustride, v[i]);
     }
 }
 template <class T, int SEG>
