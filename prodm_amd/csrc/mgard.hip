@@ -772,10 +772,22 @@ __global__ __launch_bounds__(256) void k_interp(LoadParams<T> P, const T *__rest
     const uint32_t c0s = s0 >> 1, c1s = s1 >> 1, c2s = s2 >> 1;
     const uint32_t e0 = min(c[0] - c0s, (uint32_t)kC0), e1 = min(c[1] - c1s, (uint32_t)kC1),
                    e2 = min(c[2] - c2s, (uint32_t)kC2);
-    for (uint32_t idx = threadIdx.x; idx < e0 * e1 * e2; idx += 256) {
+    // every load of the thread issued before its LDS writes (one latency,
+    // not one per element)
+    constexpr int kStage = (kC0 * kC1 * kC2 + 255) / 256;
+    const uint32_t ne = e0 * e1 * e2;
+    T sv[kStage];
+    uint32_t sl[kStage];
+#pragma unroll
+    for (int k = 0; k < kStage; k++) {
+        const uint32_t idx = threadIdx.x + 256u * k;
         const uint32_t a2 = idx % e2, a1 = (idx / e2) % e1, a0 = idx / (e2 * e1);
-        C[(a0 * kC1 + a1) * kC2 + a2] = uc[((uint64_t)(c0s + a0) * c[1] + (c1s + a1)) * c[2] + (c2s + a2)];
+        sl[k] = (a0 * kC1 + a1) * kC2 + a2;
+        sv[k] = idx < ne ? uc[((uint64_t)(c0s + a0) * c[1] + (c1s + a1)) * c[2] + (c2s + a2)] : (T)0;
     }
+#pragma unroll
+    for (int k = 0; k < kStage; k++)
+        if (threadIdx.x + 256u * k < ne) C[sl[k]] = sv[k];
     __syncthreads();
     // a wave handles rows of 64 along dim 2; every address is a tile base
     // pointer plus a 32-bit byte offset (row part + lane part)
