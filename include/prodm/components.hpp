@@ -13,6 +13,8 @@
 //   GpuPerBitBPEncoder<T, S>         MDR::PerBitBPEncoder<T, S>         (BitplaneEncoder/PerBitBPEncoder.hpp:70-403)
 //   MaxErrorCollector<T>             MDR::MaxErrorCollector<T>          (ErrorCollector/MaxErrorCollector.hpp:9-30)
 //   GpuSquaredErrorCollector<T>      MDR::SquaredErrorCollector<T>      (ErrorCollector/SquaredErrorCollector.hpp:14-70)
+//   GpuEncoderErrorCollector<T>      the encoders' level_errors overloads (NegaBinaryBPEncoder.hpp:65-125,
+//                                    GroupedBPEncoder.hpp:76-145) behind the collector interface
 //
 // The interfaces take host pointers (ProDM's components work in host
 // memory), so every call copies its operands to the device and back. The
@@ -23,6 +25,7 @@
 #ifndef PRODM_COMPONENTS_HPP
 #define PRODM_COMPONENTS_HPP
 
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <map>
@@ -488,6 +491,53 @@ class GpuSquaredErrorCollector {
     void print() const { std::printf("squared error collector (MI355X)\n"); }
 
   private:
+    Context ctx_;
+};
+
+// Squared-error tables consistent with the bitplane codec that stores the
+// level: entry j = Σ (x − decode_j(x))² with the encoder's own truncation
+// after j planes, the level_errors of NegaBinaryBPEncoder::encode
+// (NegaBinaryBPEncoder.hpp:65-125) or GroupedBPEncoder::encode
+// (GroupedBPEncoder.hpp:76-145; PerBit streams truncate the same way).
+// SquaredErrorCollector (SquaredErrorCollector.hpp:23-65) models
+// sign-magnitude truncation at the exponent of the level maximum, which for
+// negabinary streams (aligned two exponents higher, digits of both signs)
+// under-states the error of a plane count by about two planes, so an L2
+// retrieval planned on it misses its target; with these tables the planned
+// estimate is the coefficients' actual squared error. fp32 data, at most 32
+// planes, as the encoders' collectors. Same interface as the collectors.
+template <class T>
+class GpuEncoderErrorCollector {
+    static_assert(std::is_same<T, float>::value, "the encoders' level errors are defined for float data");
+
+  public:
+    explicit GpuEncoderErrorCollector(int encoder = PRODM_ENC_NEGABINARY, Context ctx = make_context())
+        : encoder_(encoder), ctx_(std::move(ctx)) {}
+    std::vector<double> collect_level_error(T const *data, size_t n, int num_bitplanes, T max_level_error) const {
+        DeviceContext &c = *ctx_;
+        T *x = c.a.as<T>(n);
+        to_device(x, data, (uint64_t)n, c.stream);
+        return collect_device(x, n, num_bitplanes, max_level_error);
+    }
+    std::vector<double> collect_device(const T *d_x, size_t n, int num_bitplanes, T max_level_error) const {
+        DeviceContext &c = *ctx_;
+        int e = 0;
+        std::frexp(max_level_error, &e);  // the exponent the encoder is given
+        std::vector<double> t(num_bitplanes + 1);
+        const size_t wsb = prodm_level_errors_workspace_bytes((uint64_t)n, num_bitplanes);
+        void *ws = c.ws.get(wsb);
+        const int rc = encoder_ == PRODM_ENC_NEGABINARY
+                           ? prodm_negabinary_level_errors_f32(d_x, n, e, num_bitplanes, t.data(), ws, wsb, c.stream)
+                           : prodm_grouped_level_errors_f32(d_x, n, e, num_bitplanes, t.data(), ws, wsb, c.stream);
+        check(rc, "encoder level errors");
+        return t;
+    }
+    void print() const {
+        std::printf("%s level-error collector (MI355X)\n", encoder_ == PRODM_ENC_NEGABINARY ? "negabinary" : "grouped");
+    }
+
+  private:
+    int encoder_;
     Context ctx_;
 };
 

@@ -36,6 +36,8 @@ namespace prodm {
 template <class C> struct is_squared_collector : std::false_type {};
 template <class T> class GpuSquaredErrorCollector;
 template <class T> struct is_squared_collector<GpuSquaredErrorCollector<T>> : std::true_type {};
+template <class T> class GpuEncoderErrorCollector;
+template <class T> struct is_squared_collector<GpuEncoderErrorCollector<T>> : std::true_type {};
 
 inline std::vector<std::vector<uint32_t>> level_dims_of(const std::vector<uint32_t> &dims, int L) {
     std::vector<uint32_t> flat((L + 1) * dims.size());

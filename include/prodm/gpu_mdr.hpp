@@ -51,16 +51,27 @@ class Plan {
     prodm_mdr_plan *p_ = nullptr;
 };
 
+// Squared-error tables a refactor stores for L2 retrieval: none, those of
+// SquaredErrorCollector (ErrorCollector/SquaredErrorCollector.hpp:23-65,
+// GpuSquaredErrorCollector), or those of the encoder's own truncation
+// (GpuEncoderErrorCollector: fp32, the tables an L2 retrieval of negabinary
+// streams must be planned on to meet its target).
+enum SquaredTables { kNoSquaredTables = 0, kCollectorTables = 1, kEncoderTables = 2 };
+
 template <class T, class Compressor, class Writer>
 class GpuRefactor {
   public:
     // encoder: PRODM_ENC_NEGABINARY (ProDM's `negabinary = true`), PRODM_ENC_GROUPED
-    // or PRODM_ENC_PERBIT;
-    // decomposer: PRODM_DECOMP_ORTHOGONAL or PRODM_DECOMP_HIERARCHICAL.
-    GpuRefactor(int encoder, Compressor compressor, Writer writer, bool collect_squared = false,
+    // or PRODM_ENC_PERBIT; squared_tables: a SquaredTables value (true =
+    // kCollectorTables); decomposer: PRODM_DECOMP_ORTHOGONAL or
+    // PRODM_DECOMP_HIERARCHICAL.
+    GpuRefactor(int encoder, Compressor compressor, Writer writer, int squared_tables = kNoSquaredTables,
                 int decomposer = PRODM_DECOMP_ORTHOGONAL)
-        : encoder_(encoder), compressor_(compressor), writer_(writer), squared_(collect_squared),
-          decomposer_(decomposer) {}
+        : encoder_(encoder), compressor_(compressor), writer_(writer), squared_(squared_tables),
+          decomposer_(decomposer) {
+        if (squared_ == kEncoderTables && !std::is_same<T, float>::value)
+            fatal("GpuRefactor: the encoder's squared-error tables are defined for float data");
+    }
 
     // Host input (copied to the device) ...
     void refactor(T const *data, const std::vector<uint32_t> &dims, uint8_t target_level, uint8_t num_bitplanes) {
@@ -104,7 +115,12 @@ class GpuRefactor {
             check(prodm_mdr_level_info(plan_->get(), l, &mx, &e, &n), "level info");
             const T tmx = (T)mx;
             m.level_error_bounds.push_back(m.negabinary ? tmx * 4 : tmx);
-            if (squared_) m.level_squared_errors.push_back(collect_squared(l, n, nb, tmx));
+            if (squared_ == kCollectorTables) m.level_squared_errors.push_back(collect_squared(l, n, nb, tmx));
+            if (squared_ == kEncoderTables) {  // the plan encoder's table (PerBit: GroupedBP's, same terms)
+                std::vector<double> t(nb + 1);
+                check(prodm_mdr_level_errors(plan_->get(), l, t.data(), stream_), "level errors");
+                m.level_squared_errors.push_back(t);
+            }
             if (grouped) {  // stream 0 carries u32 nblocks + u8 rb[nblocks] (GroupedBPEncoder.hpp:64-71)
                 uint64_t nblk = 0;
                 check(prodm_mdr_level_rb(plan_->get(), l, &rb[l], &nblk), "level rb");
@@ -238,7 +254,7 @@ class GpuRefactor {
     int encoder_;
     Compressor compressor_;
     Writer writer_;
-    bool squared_;
+    int squared_;
     int decomposer_;
     hipStream_t stream_ = nullptr;
     std::shared_ptr<Plan> plan_;

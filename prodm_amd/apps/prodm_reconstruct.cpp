@@ -52,7 +52,7 @@ static void evaluate(const std::vector<T> &u, const std::vector<double> &tols, R
             e2 += d * d;
         }
         std::printf("tolerance %.3e: reconstruct %.6fs, retrieved %zu bytes, max error %.6e (rel %.3e), "
-                    "L2 rel %.3e, PSNR %.2f\n",
+                    "L2 rel %.6e, PSNR %.2f\n",
                     t, s, r.get_retrieved_size(), linf, linf / (hi - lo), std::sqrt(e2 / n2),
                     20 * std::log10((hi - lo) / std::sqrt(e2 / u.size())));
     }

@@ -2,7 +2,8 @@
 // (counterpart of ProDM's test/test_mdr_refactor.cpp:34-86).
 //
 //   prodm_refactor <data file> <target level> <num bitplanes> <num dims> <dims...>
-//                  [--f64] [--grouped | --perbit] [--composed] [--squared] [--hierarchical] [--out DIR]
+//                  [--f64] [--grouped | --perbit] [--composed] [--squared | --squared-encoder]
+//                  [--hierarchical] [--out DIR]
 //
 // Writes DIR/metadata.bin and DIR/level_<l>.bin (default DIR refactored_data).
 // Default: float, negabinary encoder, AdaptiveLevelCompressor(64), the
@@ -10,13 +11,17 @@
 // prodm::ComposedRefactor instead (same files). --hierarchical uses the
 // hierarchical basis (ProDM's drivers use MGARDHierarchicalDecomposer);
 // --perbit the PerBit encoder (with --hierarchical, the GE applications'
-// pairing).
+// pairing). --squared stores SquaredErrorCollector's squared-error tables for
+// L2 retrieval; --squared-encoder (float) stores the encoder's own truncation
+// tables instead, which an L2 retrieval of negabinary streams needs to meet
+// its target (SquaredErrorCollector models sign-magnitude truncation).
 #include <sys/stat.h>
 
 #include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "prodm/prodm.hpp"
@@ -34,7 +39,7 @@ static std::vector<T> read_raw(const std::string &path, size_t n) {
 
 template <class T, class S>
 static int run(const std::string &file, const std::vector<uint32_t> &dims, int L, int nb, bool grouped, bool perbit,
-               bool composed, bool squared, bool hier, const std::string &dir) {
+               bool composed, int squared, bool hier, const std::string &dir) {
     const auto data = read_raw<T>(file, product(dims));
     std::vector<std::string> files;
     for (int l = 0; l <= L; l++) files.push_back(dir + "/level_" + std::to_string(l) + ".bin");
@@ -53,16 +58,19 @@ static int run(const std::string &file, const std::vector<uint32_t> &dims, int L
             if (hier) go2(GpuMGARDHierarchicalDecomposer<T>(ctx), encoder, collector);
             else go2(GpuMGARDOrthogonalDecomposer<T>(ctx), encoder, collector);
         };
-        if (perbit) {
-            if (squared) go(GpuPerBitBPEncoder<T, S>(ctx), GpuSquaredErrorCollector<T>(ctx));
-            else go(GpuPerBitBPEncoder<T, S>(ctx), MaxErrorCollector<T>());
-        } else if (grouped) {
-            if (squared) go(GpuGroupedBPEncoder<T, S>(ctx), GpuSquaredErrorCollector<T>(ctx));
-            else go(GpuGroupedBPEncoder<T, S>(ctx), MaxErrorCollector<T>());
-        } else {
-            if (squared) go(GpuNegaBinaryBPEncoder<T, S>(ctx), GpuSquaredErrorCollector<T>(ctx));
-            else go(GpuNegaBinaryBPEncoder<T, S>(ctx), MaxErrorCollector<T>());
-        }
+        auto with = [&](auto encoder, int enc) {
+            if (squared == kCollectorTables) {
+                go(encoder, GpuSquaredErrorCollector<T>(ctx));
+            } else if (squared == kEncoderTables) {
+                if constexpr (std::is_same<T, float>::value) go(encoder, GpuEncoderErrorCollector<T>(enc, ctx));
+                else fatal("--squared-encoder needs float data");
+            } else {
+                go(encoder, MaxErrorCollector<T>());
+            }
+        };
+        if (perbit) with(GpuPerBitBPEncoder<T, S>(ctx), PRODM_ENC_PERBIT);
+        else if (grouped) with(GpuGroupedBPEncoder<T, S>(ctx), PRODM_ENC_GROUPED);
+        else with(GpuNegaBinaryBPEncoder<T, S>(ctx), PRODM_ENC_NEGABINARY);
     } else {
         GpuRefactor<T, AdaptiveLevelCompressor, ConcatLevelFileWriter> r(
             perbit ? PRODM_ENC_PERBIT : grouped ? PRODM_ENC_GROUPED : PRODM_ENC_NEGABINARY, AdaptiveLevelCompressor(64), writer, squared,
@@ -81,7 +89,8 @@ static int run(const std::string &file, const std::vector<uint32_t> &dims, int L
 int main(int argc, char **argv) {
     if (argc < 6) {
         std::fprintf(stderr, "usage: %s <file> <target level> <bitplanes> <ndims> <dims...> [--f64] [--grouped] "
-                             "[--perbit] [--composed] [--squared] [--hierarchical] [--out DIR]\n", argv[0]);
+                             "[--perbit] [--composed] [--squared | --squared-encoder] [--hierarchical] [--out DIR]\n",
+                     argv[0]);
         return 2;
     }
     int a = 1;
@@ -91,7 +100,8 @@ int main(int argc, char **argv) {
     const int nd = std::atoi(argv[a++]);
     std::vector<uint32_t> dims;
     for (int i = 0; i < nd && a < argc; i++) dims.push_back((uint32_t)std::atoi(argv[a++]));
-    bool f64 = false, grouped = false, perbit = false, composed = false, squared = false, hier = false;
+    bool f64 = false, grouped = false, perbit = false, composed = false, hier = false;
+    int squared = kNoSquaredTables;
     std::string dir = "refactored_data";
     for (; a < argc; a++) {
         const std::string o = argv[a];
@@ -99,7 +109,8 @@ int main(int argc, char **argv) {
         else if (o == "--grouped") grouped = true;
         else if (o == "--perbit") perbit = true;
         else if (o == "--composed") composed = true;
-        else if (o == "--squared") squared = true;
+        else if (o == "--squared") squared = kCollectorTables;
+        else if (o == "--squared-encoder") squared = kEncoderTables;
         else if (o == "--hierarchical") hier = true;
         else if (o == "--out" && a + 1 < argc) dir = argv[++a];
         else fatal("unknown option " + o);

@@ -6,8 +6,9 @@
 //  2. prodm::ComposedReconstructor and prodm::GpuReconstructor read them
 //     progressively; each result meets the requested L-infinity tolerance
 //     and the two agree to rounding.
-//  3. The squared-error tables written with a squared collector drive an
-//     S-norm reconstruction whose actual relative L2 error is reported.
+//  3. The negabinary encoder's squared-error tables (composed and
+//     device-resident refactors write the same files) drive S-norm
+//     reconstructions that meet each relative L2 target 1e-2 .. 1e-6.
 //  4. GPU components against each other: encode -> progressive decode
 //     equals one-shot decode (negabinary, grouped, per-bit); interleave ->
 //     reposition is the identity.
@@ -147,30 +148,69 @@ static void roundtrip_files(const std::string &root, const std::vector<float> &u
 
 static void squared_path(const std::string &root, const std::vector<float> &u, const std::vector<uint32_t> &dims, int L,
                          int nb) {
-    const std::string a = root + "/gpu_sq";
+    // Tables of the negabinary encoder's own truncation (GpuEncoderErrorCollector):
+    // the composed refactor with that collector and GpuRefactor(kEncoderTables)
+    // write the same files, and an S-norm retrieval planned on them meets each
+    // relative L2 target. ProDM's SquaredErrorCollector tables (the composed
+    // refactor with GpuSquaredErrorCollector) are stored as well; planned on
+    // them, negabinary retrievals can miss the target, so those results are
+    // only reported.
+    const std::string a = root + "/composed_sq", b = root + "/gpu_sq", c = root + "/gpu_sqc";
     mkdir(a.c_str(), 0755);
-    GpuRefactor<float, DefaultLevelCompressor, ConcatLevelFileWriter> r(
-        PRODM_ENC_NEGABINARY, DefaultLevelCompressor(), ConcatLevelFileWriter(a + "/metadata.bin", level_files(a, L)),
-        true);
-    r.refactor(u.data(), dims, L, nb);
+    mkdir(b.c_str(), 0755);
+    mkdir(c.c_str(), 0755);
+    Context ctx = make_context();
+    {
+        ComposedRefactor<float, GpuMGARDOrthogonalDecomposer<float>, GpuDirectInterleaver<float>,
+                         GpuNegaBinaryBPEncoder<float, uint32_t>, DefaultLevelCompressor, GpuEncoderErrorCollector<float>,
+                         ConcatLevelFileWriter>
+            r{GpuMGARDOrthogonalDecomposer<float>(ctx), GpuDirectInterleaver<float>(ctx),
+              GpuNegaBinaryBPEncoder<float, uint32_t>(ctx), DefaultLevelCompressor(),
+              GpuEncoderErrorCollector<float>(PRODM_ENC_NEGABINARY, ctx),
+              ConcatLevelFileWriter(a + "/metadata.bin", level_files(a, L))};
+        r.negabinary = true;
+        r.refactor(u.data(), dims, L, nb);
+    }
+    {
+        GpuRefactor<float, DefaultLevelCompressor, ConcatLevelFileWriter> r(
+            PRODM_ENC_NEGABINARY, DefaultLevelCompressor(), ConcatLevelFileWriter(b + "/metadata.bin", level_files(b, L)),
+            kEncoderTables);
+        r.refactor(u.data(), dims, L, nb);
+    }
+    {
+        GpuRefactor<float, DefaultLevelCompressor, ConcatLevelFileWriter> r(
+            PRODM_ENC_NEGABINARY, DefaultLevelCompressor(), ConcatLevelFileWriter(c + "/metadata.bin", level_files(c, L)),
+            kCollectorTables);
+        r.refactor(u.data(), dims, L, nb);
+    }
+    EXPECT(slurp(a + "/metadata.bin") == slurp(b + "/metadata.bin") && !slurp(a + "/metadata.bin").empty(),
+           "squared-table metadata differs (composed vs device-resident)");
+    for (int l = 0; l <= L; l++)
+        EXPECT(slurp(level_files(a, L)[l]) == slurp(level_files(b, L)[l]), "squared-table level %d file differs", l);
     using Est = SNormErrorEstimator<float>;
     using Interp = NegaBinaryGreedyBasedSizeInterpreter<Est>;
-    GpuReconstructor<float, DefaultLevelCompressor, Interp, Est, ConcatLevelFileRetriever> rg(
-        DefaultLevelCompressor(), Interp(Est(3, L, 0)), ConcatLevelFileRetriever(a + "/metadata.bin", level_files(a, L)));
-    rg.load_metadata();
     double norm2 = 0;
     for (float v : u) norm2 += (double)v * v;
-    for (double rel : {1e-2, 1e-4, 1e-6}) {
-        // the S-norm estimate is in units of squared coefficient error
-        // times level volume; ask for (rel * ||u||)^2 over the grid
-        const double tol = rel * rel * norm2;
-        const float *g = rg.progressive_reconstruct(tol);
-        EXPECT(g != nullptr, "no squared tables");
-        if (!g) return;
-        double e2 = 0;
-        for (size_t i = 0; i < u.size(); i++) e2 += ((double)u[i] - g[i]) * ((double)u[i] - g[i]);
-        std::printf("L2 target %.0e: actual relative L2 %.3e, retrieved %zu bytes\n", rel, std::sqrt(e2 / norm2),
-                    rg.get_retrieved_size());
+    for (const std::string &dir : {b, c}) {
+        GpuReconstructor<float, DefaultLevelCompressor, Interp, Est, ConcatLevelFileRetriever> rg(
+            DefaultLevelCompressor(), Interp(Est(3, L, 0)),
+            ConcatLevelFileRetriever(dir + "/metadata.bin", level_files(dir, L)));
+        rg.load_metadata();
+        const bool encoder_tables = dir == b;
+        for (double rel : {1e-2, 1e-3, 1e-4, 1e-5, 1e-6}) {
+            // the S-norm estimate sums squared coefficient errors weighted by
+            // level volume, in units of the fine grid: ask for (rel ||u||)^2
+            const double tol = rel * rel * norm2;
+            const float *g = rg.progressive_reconstruct(tol);
+            EXPECT(g != nullptr, "no squared tables");
+            if (!g) return;
+            double e2 = 0;
+            for (size_t i = 0; i < u.size(); i++) e2 += ((double)u[i] - g[i]) * ((double)u[i] - g[i]);
+            const double got = std::sqrt(e2 / norm2);
+            std::printf("L2 target %.0e (%s tables): actual relative L2 %.3e, retrieved %zu bytes\n", rel,
+                        encoder_tables ? "encoder" : "SquaredErrorCollector", got, rg.get_retrieved_size());
+            if (encoder_tables) EXPECT(got <= rel, "relative L2 %.3e above the target %.0e", got, rel);
+        }
     }
 }
 
@@ -196,7 +236,7 @@ static void hierarchical_path(const std::string &root, const std::vector<float> 
     {
         GpuRefactor<float, AdaptiveLevelCompressor, ConcatLevelFileWriter> r(
             PRODM_ENC_NEGABINARY, AdaptiveLevelCompressor(64), ConcatLevelFileWriter(b + "/metadata.bin", level_files(b, L)),
-            false, PRODM_DECOMP_HIERARCHICAL);
+            kNoSquaredTables, PRODM_DECOMP_HIERARCHICAL);
         r.refactor(u.data(), dims, L, nb);
     }
     EXPECT(slurp(a + "/metadata.bin") == slurp(b + "/metadata.bin"), "hierarchical metadata differs");
@@ -229,7 +269,7 @@ static void perbit_path(const std::string &root, const std::vector<float> &u, co
     {  // the device-resident path with the PerBit encoder writes the same files
         GpuRefactor<float, AdaptiveLevelCompressor, ConcatLevelFileWriter> r(
             PRODM_ENC_PERBIT, AdaptiveLevelCompressor(64), ConcatLevelFileWriter(b + "/metadata.bin", level_files(b, L)),
-            false, PRODM_DECOMP_HIERARCHICAL);
+            kNoSquaredTables, PRODM_DECOMP_HIERARCHICAL);
         r.refactor(u.data(), dims, L, nb);
     }
     {

@@ -30,8 +30,15 @@ def test_cpp_pipeline(gpu, tmp_path):
     assert "PASSED" in r.stdout
 
 
-@pytest.mark.parametrize("mode", [[], ["--grouped"], ["--composed"], ["--squared"], ["--hierarchical"]])
+@pytest.mark.parametrize("mode", [[], ["--grouped"], ["--composed"], ["--squared"], ["--squared-encoder"],
+                                  ["--squared-encoder", "--composed"], ["--hierarchical"]])
 def test_apps_refactor_reconstruct(gpu, tmp_path, mode):
+    """Refactor a 65x65x33 field into files and reconstruct it progressively.
+    L-inf tolerances 1e-1, 1e-3, 1e-5 of the range are met; with the
+    negabinary encoder's squared-error tables (--squared-encoder), S-norm
+    retrievals at relative L2 1e-2 .. 1e-6 meet each target. (Files written
+    with --squared carry SquaredErrorCollector's tables, which model
+    sign-magnitude truncation; they are reconstructed to L-inf tolerances.)"""
     dims = [65, 65, 33]
     u = bumps_field(dims, seed=5)
     f = tmp_path / "field.f32"
@@ -40,16 +47,25 @@ def test_apps_refactor_reconstruct(gpu, tmp_path, mode):
     r = run([os.path.join(BIN, "prodm_refactor"), str(f), "4", "32", "3", *map(str, dims), "--out", str(out), *mode])
     assert r.returncode == 0
     R = float(u.max() - u.min())
-    tols = [1e-1 * R, 1e-3 * R, 1e-5 * R]
-    extra = ["--l2"] if mode == ["--squared"] else (["--hierarchical"] if mode == ["--hierarchical"] else [])
-    if extra == ["--l2"]:
-        tols = [(t * float(np.linalg.norm(u))) ** 2 for t in (1e-2, 1e-4)]
+    norm2 = float(np.sum(u.astype(np.float64) ** 2))
+    l2 = "--squared-encoder" in mode
+    if l2:
+        rels = [1e-2, 1e-3, 1e-4, 1e-5, 1e-6]
+        tols = [rel * rel * norm2 for rel in rels]  # S-norm estimate: squared error on the fine grid
+        extra = ["--l2"]
+    else:
+        tols = [1e-1 * R, 1e-3 * R, 1e-5 * R]
+        extra = ["--hierarchical"] if "--hierarchical" in mode else []
     r = run([os.path.join(BIN, "prodm_reconstruct"), str(f), str(len(tols)), *map(repr, tols), "--in", str(out),
              *extra])
     assert r.returncode == 0
     lines = [l for l in r.stdout.splitlines() if l.startswith("tolerance")]
     assert len(lines) == len(tols)
-    if extra != ["--l2"]:
+    if l2:
+        for rel, line in zip(rels, lines):
+            got = float(line.split("L2 rel ")[1].split(",")[0])
+            assert got <= rel, line
+    else:
         for t, line in zip(tols, lines):
             err = float(line.split("max error ")[1].split()[0])
             assert err <= t, line

@@ -12,7 +12,8 @@ oracle's own formulas:
 * interleave/reposition: brute force over the box difference.
 * codecs: truncation identities, stream sizes from an independent count of
   recording planes, hand-built single-bit blocks.
-* squared collector: exact rational sum of the per-element terms.
+* squared collector: exact rational sum of the per-element terms, and the
+  table as the squared error of sign-magnitude truncation (fp32 and fp64).
 """
 import itertools
 import math
@@ -346,6 +347,77 @@ def test_squared_collector_is_exactly_rounded(nb):
     want = [float(sum((Fraction(float(t[k])) for t in terms), Fraction(0))) for k in range(nb + 1)]
     got = ob.collect_squared(x, nb, m, mode="exact")
     np.testing.assert_array_equal(got, np.array(want))
+
+
+def truncation_table(x, maxerr, nb, dt):
+    """SquaredErrorCollector's table from what it measures, without its
+    mantissa-bit loop (SquaredErrorCollector.hpp:23-65): entry k is the sum
+    over the elements of the squared error of sign-magnitude truncation to
+    the weight 2^(le - k), le the exponent of maxerr (maxerr in
+    [2^(le-1), 2^le)): T_k(x) = sign(x) floor(|x| 2^(k - le)) 2^(le - k).
+    The residual d = x - T_k(x) is exact in the data type; its square is
+    rounded once in the data type (the collector squares in T) and the terms
+    are summed exactly, then rounded to double. Normal, finite x only."""
+    le = math.frexp(float(maxerr))[1]
+    tot = [Fraction(0)] * (nb + 1)
+    for v in x:
+        f = Fraction(float(v))
+        if f == 0:
+            continue
+        a = abs(f)
+        for k in range(nb + 1):
+            s = Fraction(2) ** (k - le)
+            d = a - Fraction(math.floor(a * s)) / s
+            dT = dt(float(d))
+            assert Fraction(float(dT)) == d  # the residual is exact in T
+            tot[k] += Fraction(float(dT * dT))
+    return np.array([float(t) for t in tot])
+
+
+def squared_x(n, dt, seed, spread):
+    """Normal numbers of both signs spread over 2^-spread .. 1, zeros and
+    exact powers of two among them."""
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0.5, 1.0, n) * np.exp2(-rng.integers(0, spread, n)) * rng.choice([-1.0, 1.0], n)
+    x[rng.random(n) < 0.05] = 0
+    x[rng.random(n) < 0.05] = np.exp2(-rng.integers(0, spread, 1))[0]
+    return x.astype(dt)
+
+
+@pytest.mark.parametrize("dt,nb,spread", [(np.float64, 64, 70), (np.float64, 60, 40), (np.float64, 40, 90),
+                                          (np.float64, 20, 12), (np.float32, 32, 40), (np.float32, 16, 10)])
+def test_squared_collector_is_truncation_error(dt, nb, spread):
+    """The collector's table (exact and sequential-order sums) against the
+    truncation closed form. For fp64 it pins the 64-bit mantissa mask that
+    the oracle uses where the reference shifts a 32-bit 1u by up to 51 (B3):
+    a 32-bit mask would leave bits 32..51 set and miss those residuals.
+    Elements far below 2^(le - nb) (spread > nb) give x^2 to every entry."""
+    x = squared_x(200, dt, nb + spread, spread)
+    m = dt(np.max(np.abs(x)))
+    want = truncation_table(x, m, nb, dt)
+    np.testing.assert_array_equal(ob.collect_squared(x, nb, m, mode="exact"), want)
+    seq = ob.collect_squared(x, nb, m, mode="seq")
+    np.testing.assert_allclose(seq, want, rtol=x.size * 2.0 ** -52, atol=0)
+    # the sequential mode adds the float terms in element order, in double
+    terms = [truncation_table(x[i:i + 1], m, nb, dt) for i in range(0, x.size, 9)]
+    seq9 = ob.collect_squared(np.ascontiguousarray(x[::9]), nb, m, mode="seq")
+    acc = np.zeros(nb + 1)
+    for t in terms:
+        acc = acc + t
+    np.testing.assert_array_equal(seq9, acc)
+
+
+def test_squared_collector_fp64_kat():
+    """One element, by hand: x = 1 + 2^-40 (bit 12 of the 52-bit mantissa
+    field set), maxerr = x (le = 1), nb = 52: T_k keeps
+    weights >= 2^(1-k). Entries k <= 40 hold (2^-40)^2 = 2^-80 (the
+    implicit 1 is kept from k = 1 on, and k = 0 keeps nothing: x^2);
+    entries k >= 41 hold 0, since 2^-40 = 2^(1-41) is kept from k = 41."""
+    x = np.array([1.0 + 2.0 ** -40])
+    t = ob.collect_squared(x, 52, x[0], mode="exact")
+    assert t[0] == float(Fraction(1 + Fraction(1, 2 ** 40)) ** 2)
+    assert all(t[k] == 2.0 ** -80 for k in range(1, 41))
+    assert all(t[k] == 0.0 for k in range(41, 53))
 
 
 # ---------------------------------------------------------------- PerBit ---
