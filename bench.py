@@ -275,6 +275,28 @@ def end_to_end_report(n=513, L=6, nb=32, reps=3, tols=(1e-2, 1e-4, 1e-6)):
     return out
 
 
+def qoi_report(n=257, L=5, nb=32, rel=1e-3):
+    """The adjoint-guided QoI workload: |V| of a divergence-free n^3 float32
+    velocity (24 Fourier modes, |k| <= 32, seeds 10, 11, 12), L levels, nb
+    planes, NegaBinary, |V| L-inf tolerance rel * range(|V|). Plane bytes
+    read by ProDM's halving loop (halving_Vtot.cpp) and by the adjoint-guided
+    planner, each checked against the original field (bound over every node
+    and actual error at most tau)."""
+    from prodm_amd import qoi
+    log(f"[bench] QoI: |V| of a {n}^3 velocity, L={L}, nb={nb}, rel {rel:g} ...")
+    t0 = time.perf_counter()
+    res = qoi.run(n, L, nb, rel)
+    res["seconds"] = round(time.perf_counter() - t0, 2)
+    for k in ("plain", "adjoint_guided"):
+        r = res[k]
+        if not (r["max_error"] <= r["tau"] and r["max_bound"] <= r["tau"]):
+            log(f"[bench] FAIL: QoI {k} misses tau: {r}")
+            sys.exit(1)
+    log(f"[bench] QoI bytes: plain {res['plain']['bytes']}, adjoint-guided {res['adjoint_guided']['bytes']} "
+        f"({100 * res['bytes_reduction']:.1f}% fewer)")
+    return res
+
+
 def plan_planes(level_bounds, level_plane_bytes, nb, tol):
     """Planes per level for an L-infinity tolerance: the host C++ greedy
     planner (GreedyBasedSizeInterpreter.hpp:23-70) over the MaxErrorCollector
@@ -367,6 +389,8 @@ def main():
     ap.add_argument("--batched-only", action="store_true", help="run only the batched report (one JSON line)")
     ap.add_argument("--no-perbit", action="store_true", help="skip the PerBit encode report")
     ap.add_argument("--perbit-only", action="store_true", help="run only the PerBit encode report (one JSON line)")
+    ap.add_argument("--no-qoi", action="store_true", help="skip the adjoint-guided QoI retrieval report")
+    ap.add_argument("--qoi-only", action="store_true", help="run only the QoI retrieval report (one JSON line)")
     ap.add_argument("--end-to-end", action="store_true",
                     help="run only the host-buffer + files report of the C++ applications (one JSON line)")
     ap.add_argument("--calibrate", action="store_true",
@@ -391,6 +415,9 @@ def main():
         return
     if args.end_to_end:
         print(json.dumps(end_to_end_report()), flush=True)
+        return
+    if args.qoi_only:
+        print(json.dumps(qoi_report()), flush=True)
         return
 
     n, L, nb = args.size, args.levels, args.bitplanes
@@ -586,6 +613,8 @@ def main():
         res["batched"] = batched_report(args.steps, args.warmup)
     if not args.no_perbit:
         res["perbit"] = perbit_report(args.steps, args.warmup)
+    if not args.no_qoi:
+        res["qoi"] = qoi_report()
     print(json.dumps(res), flush=True)
 
 

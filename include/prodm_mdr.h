@@ -306,6 +306,32 @@ int prodm_interpret_retrieve_size(int planner, int estimator, int dtype, int ndi
  * arithmetic; 0 for ndims outside 1..3.                                    */
 double prodm_max_estimator_ob_constant(int ndims, int dtype);
 
+/* ---- certified amplification and QoI planning (host only; C++ in
+ *      include/prodm/qoi.hpp). out[l], l = 0..target_level: the largest
+ *      row 1-norm, over every node, of the level-l block of the recompose
+ *      operator (exact, from the 1-D L2 projections): a retrieval whose
+ *      level-l coefficient errors are at most t_l is within sum_l out[l] t_l
+ *      of the field at every node. It replaces the one constant per
+ *      dimension count of MaxErrorEstimatorOB (MaxErrorEstimator.hpp:13-40).
+ *      decomposer: PRODM_DECOMP_ORTHOGONAL or PRODM_DECOMP_HIERARCHICAL
+ *      (all 1).                                                            */
+int prodm_recompose_amplification(const uint32_t *dims, int ndims, int target_level, int decomposer, double *out);
+/* Greedy planning of several components' planes on a quantity of interest
+ * (QoIGreedyBasedSizeInterpreter): the interpreter arguments of
+ * prodm_interpret_retrieve_size over ncomp * nlevels (component, level)
+ * entries, component-major; errors = per-coefficient error bounds after
+ * each plane count (MaxErrorCollector tables); amplification[nlevels] as
+ * prodm_recompose_amplification; rounding[ncomp] (NULL: 0) added to each
+ * component's bound; sample[npoints * ncomp] reconstructed values (point
+ * major) at which the QoI bound is evaluated (ProDM's composition,
+ * include/qoi_utils.hpp:18-41). Planes are added until the sampled bound is
+ * below the tolerance; index is updated, out_sizes = bytes to read.       */
+#define PRODM_QOI_VELOCITY_MAGNITUDE 0
+int prodm_qoi_interpret_retrieve_size(int qoi, int ncomp, int nlevels, const double *amplification,
+                                      const double *rounding, const int *counts, const uint32_t *sizes,
+                                      const double *errors, const double *sample, uint64_t npoints, double tolerance,
+                                      uint8_t *index, uint32_t *out_sizes);
+
 /* ---- batched refactor (no counterpart in ProDM, which refactors one field
  *      per ComposedRefactor::refactor call, ComposedRefactor.hpp:78-141):
  *      many fields of uneven shapes refactored together with one grouped

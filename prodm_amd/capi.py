@@ -14,6 +14,7 @@ LIB_PATH = os.environ.get("PRODM_MDR_LIB") or os.path.join(HERE, "libprodm_mdr.s
 F32, F64 = 0, 1
 ORTHOGONAL, HIERARCHICAL = 0, 1
 NEGABINARY, GROUPED, PERBIT = 0, 1, 2
+QOI_VELOCITY_MAGNITUDE = 0
 
 _lib = None
 
@@ -86,6 +87,9 @@ _SIGS = {
     "prodm_interpret_retrieve_size": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
                                                 vp, vp, vp, C.c_double, vp, vp]),
     "prodm_max_estimator_ob_constant": (C.c_double, [C.c_int, C.c_int]),
+    "prodm_recompose_amplification": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp]),
+    "prodm_qoi_interpret_retrieve_size": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, u64,
+                                                    C.c_double, vp, vp]),
     "prodm_level_errors_workspace_bytes": (sz, [u64, C.c_int]),
     "prodm_negabinary_level_errors_f32": (C.c_int, [vp, u64, i32, C.c_int, vp, vp, sz, vp]),
     "prodm_grouped_level_errors_f32": (C.c_int, [vp, u64, i32, C.c_int, vp, vp, sz, vp]),

@@ -10,6 +10,7 @@
 
 #include "prodm/estimators.hpp"
 #include "prodm/interpreters.hpp"
+#include "prodm/qoi.hpp"
 #include "prodm_mdr.h"
 
 namespace prodm {
@@ -111,4 +112,67 @@ extern "C" double prodm_max_estimator_ob_constant(int ndims, int dtype) {
     if (ndims < 1 || ndims > 3) return 0.0;
     return dtype == PRODM_F64 ? prodm::MaxErrorEstimatorOB<double>(ndims).constant()
                               : (double)prodm::MaxErrorEstimatorOB<float>(ndims).constant();
+}
+
+extern "C" int prodm_recompose_amplification(const uint32_t *dims, int ndims, int target_level, int decomposer,
+                                             double *out) {
+    if (!dims || !out || ndims < 1 || ndims > 3 || target_level < 0 ||
+        (decomposer != PRODM_DECOMP_ORTHOGONAL && decomposer != PRODM_DECOMP_HIERARCHICAL)) {
+        fail("prodm_recompose_amplification: bad arguments");
+        return PRODM_ERR_ARG;
+    }
+    std::vector<uint32_t> flat((target_level + 1) * ndims);
+    std::vector<uint64_t> el(target_level + 1);
+    if (!prodm_level_geometry(dims, ndims, target_level, flat.data(), el.data())) {
+        fail("prodm_recompose_amplification: dims do not stay odd down to level 0");
+        return PRODM_ERR_GEOMETRY;
+    }
+    try {
+        const auto a = prodm::recompose_amplification(std::vector<uint32_t>(dims, dims + ndims), target_level,
+                                                      decomposer == PRODM_DECOMP_HIERARCHICAL);
+        std::memcpy(out, a.data(), sizeof(double) * a.size());
+    } catch (const std::exception &ex) {
+        fail("prodm_recompose_amplification: %s", ex.what());
+        return PRODM_ERR_ARG;
+    }
+    return PRODM_OK;
+}
+
+extern "C" int prodm_qoi_interpret_retrieve_size(int qoi, int ncomp, int nlevels, const double *amplification,
+                                                 const double *rounding, const int *counts, const uint32_t *sizes,
+                                                 const double *errors, const double *sample, uint64_t npoints,
+                                                 double tolerance, uint8_t *index, uint32_t *out_sizes) {
+    if (qoi != PRODM_QOI_VELOCITY_MAGNITUDE || ncomp < 1 || nlevels < 1 || !amplification || !counts || !sizes ||
+        !errors || (npoints && !sample) || !index || !out_sizes) {
+        fail("prodm_qoi_interpret_retrieve_size: bad arguments");
+        return PRODM_ERR_ARG;
+    }
+    const int nk = ncomp * nlevels;
+    prodm::LevelSizes s(nk);
+    prodm::LevelErrors e(nk);
+    std::vector<uint8_t> idx(index, index + nk);
+    for (int k = 0; k < nk; k++) {
+        if (counts[k] < 0 || counts[k] > 255 || idx[k] > counts[k]) {
+            fail("prodm_qoi_interpret_retrieve_size: entry %d: bad plane count or index", k);
+            return PRODM_ERR_ARG;
+        }
+        s[k].assign(sizes, sizes + counts[k]);
+        e[k].assign(errors, errors + counts[k] + 1);
+        sizes += counts[k];
+        errors += counts[k] + 1;
+    }
+    std::vector<uint32_t> out;
+    try {
+        prodm::QoIGreedyBasedSizeInterpreter<prodm::VelocityMagnitude> q(
+            ncomp, std::vector<double>(amplification, amplification + nlevels),
+            rounding ? std::vector<double>(rounding, rounding + ncomp) : std::vector<double>());
+        q.set_sample(std::vector<double>(sample, sample + npoints * ncomp));
+        out = q.interpret_retrieve_size(s, e, tolerance, idx);
+    } catch (const std::exception &ex) {
+        fail("prodm_qoi_interpret_retrieve_size: %s", ex.what());
+        return PRODM_ERR_ARG;
+    }
+    std::memcpy(index, idx.data(), nk);
+    std::memcpy(out_sizes, out.data(), sizeof(uint32_t) * nk);
+    return PRODM_OK;
 }

@@ -2,30 +2,34 @@
 from three refactored components, against the halving loop of ProDM's GE
 application (app/GE/sequential/halving_Vtot.cpp:31-84,149-189).
 
-Error model. For one component, a retrieval that holds k_l planes of level l
-leaves coefficient errors |dc| <= t_l = 2^(e_l + 2 - k_l) + 2^(e_l + 2 - nb)
-(negabinary truncation; t_l = max |c| for k_l = 0), so the field error at a
-point x is bounded by
-    |dV(x)| <= sum_l a_l(x) t_l,   a_l(x) = || row x of R, level-l part ||_1
-with R the recompose operator. a_l(x) is read off the adjoint: R^T e_x is the
-row of x. The QoI error follows ProDM's composition of bounds
-(include/qoi_utils.hpp:18-41): e(|V|^2) <= sum_i 2 |V_i| E_i + E_i^2 and
-e(|V|) <= e2 / (sqrt(|V|^2 - e2) + |V|) (or e2 / |V|, sqrt(e2) at 0).
+Error model. A retrieval holding k planes of a level leaves every
+coefficient's error below t_l[k], the MaxErrorCollector table of the level's
+stored bound (MaxErrorCollector.hpp:14-25; for negabinary streams the bound
+is 4 max|c|, so t_l[k] = 2^(e_l + 2 - k) for k >= 1, which bounds the
+negabinary truncation, and t_l[0] >= max|c|). The field error at every node
+is then at most E = sum_l A_l t_l[k_l] + r, A_l the exact largest row
+1-norm of recompose's level-l block (prodm_recompose_amplification,
+include/prodm/qoi.hpp) and r an allowance for rounding: 2^-21 of each level
+maximum for the decoder's float conversion (amplified by A_l) and 2^-19 of
+the component's magnitude per level for the float32 recompose. The QoI error
+follows ProDM's composition of bounds (include/qoi_utils.hpp:18-41):
+e(|V|^2) <= sum_i 2 |V~_i| E_i + E_i^2 and e(|V|) <= e2 / (sqrt(|V~|^2 - e2)
++ |V~|) (or e2 / |V~|, sqrt(e2) at 0), at the reconstructed values V~.
 
 * plain: ProDM's loop. Each component is retrieved to an L-inf tolerance
   eps_i (greedy planner, MaxErrorEstimatorOB); the QoI bound is evaluated
   with E_i = eps_i at every point; while its maximum exceeds tau, all eps_i
   shrink by 1.5 (halving_Vtot.cpp:56-70) and the components are retrieved
-  again.
-* adjoint-guided: the per-level amplification A_l = max a_l(x) over probe
-  points (interior parity classes and the boundary), read off adjoint rows,
-  replaces the estimator constant, and one greedy over all (component,
-  level) planes adds the plane that lowers the largest QoI bound (composed
-  at the reconstructed field, so each component counts by its sensitivity
-  |V_i| / |V|) the most per byte, until the bound holds at every point.
+  again. MaxErrorEstimatorOB's constant bounds A_l, so eps_i bounds the
+  component error.
+* adjoint-guided: each round retrieves, evaluates the certified bound over
+  every node, and hands the nodes where it is largest to the QoI greedy
+  planner (QoIGreedyBasedSizeInterpreter, one plane of one component's level
+  at a time, the largest drop of the bound per byte first), until the bound
+  holds at every node.
 
-Both report the plane bytes read and the actual QoI error against the
-original field.
+Both report the plane bytes read, the largest certified bound over the field
+and the actual QoI error against the original field.
 """
 import math
 
@@ -93,12 +97,25 @@ class Component:
         self.plane_bytes = [((n + 31) // 32) * 4 for _, _, n in self.info]
         self.k = [0] * (L + 1)
 
-    def t(self, l, k):
-        """Bound on the coefficient error of level l holding k planes."""
-        m, e, _ = self.info[l]
-        if k == 0:
-            return float(m)
-        return math.ldexp(1.0, e + 2 - k) + math.ldexp(1.0, e + 2 - self.nb)
+    def tables(self):
+        """Per level, the MaxErrorCollector table of the stored bound (4 max
+        for negabinary): entry k bounds every coefficient's error after k
+        planes."""
+        import ctypes as C
+        out = []
+        for m, _, _ in self.info:
+            t = np.zeros(self.nb + 1)
+            capi.lib().prodm_collect_max_f32(C.c_float(np.float32(m) * np.float32(4.0)), self.nb,
+                                             t.ctypes.data_as(C.c_void_p))
+            out.append(t)
+        return out
+
+    def rounding(self, A, vmax):
+        """Allowance for the decoder's float conversion (2^-21 of each level
+        maximum, amplified) and the float32 recompose (2^-19 of the
+        component's magnitude per level)."""
+        dec = sum(A[l] * math.ldexp(float(m), -21) for l, (m, _, _) in enumerate(self.info))
+        return dec + (self.L + 1) * math.ldexp(float(vmax), -19)
 
     def bytes(self):
         return sum(k * b for k, b in zip(self.k, self.plane_bytes))
@@ -110,60 +127,11 @@ class Component:
         """Planes for an L-inf tolerance: the greedy planner over
         MaxErrorCollector tables with MaxErrorEstimatorOB(3), continuing from
         the planes held (ComposedReconstructor::progressive_reconstruct)."""
-        import ctypes as C
-        tables = []
-        for m, _, _ in self.info:
-            t = np.zeros(self.nb + 1)
-            capi.lib().prodm_collect_max_f32(C.c_float(np.float32(m) * np.float32(4.0)), self.nb,
-                                             t.ctypes.data_as(C.c_void_p))
-            tables.append(t)
+        tables = self.tables()
         sizes = [[b] * self.nb for b in self.plane_bytes]
         idx = np.array(self.k, np.uint8)
         _, idx = interpret.retrieve_sizes(interpret.GREEDY, interpret.MAX_OB, sizes, tables, tol, idx, ndims=3)
         self.k = [int(v) for v in idx]
-
-
-# ---- adjoint rows ------------------------------------------------------------------
-def level_boxes(dims, L):
-    ld, _, _ = mdr.level_geometry(dims, L)
-    return [tuple(int(v) for v in ld[l]) for l in range(L + 1)]
-
-
-def row_level_norms(dims, L, points, device):
-    """a_l(x) = L1 norm of the level-l part of row x of the recompose
-    operator, from one adjoint per point: returns (len(points), L+1)."""
-    boxes = level_boxes(dims, L)
-    N = int(np.prod(dims))
-    out = torch.zeros(len(points), L + 1, dtype=torch.float64, device=device)
-    e = torch.zeros(N, dtype=torch.float64, device=device)
-    for p, x in enumerate(points):
-        e.zero_()
-        e[x] = 1.0
-        r = mdr.recompose_adjoint(e, dims, L).abs().reshape(dims)
-        prev = 0.0
-        for l in range(L + 1):
-            b = boxes[l]
-            s = float(r[:b[0], :b[1], :b[2]].sum())
-            out[p, l] = s - prev
-            prev = s
-    return out
-
-
-def probe_points(dims):
-    """Points covering the interior parity classes and the boundary: the 27
-    parity/position combinations around the centre, and faces, edges and
-    corners."""
-    pts = set()
-    c = [n // 2 for n in dims]
-    for d0 in range(3):
-        for d1 in range(3):
-            for d2 in range(3):
-                pts.add((c[0] + d0, c[1] + d1, c[2] + d2))
-    for a in (0, 1, 2, dims[0] // 2, dims[0] - 2, dims[0] - 1):
-        for b in (0, 1, dims[1] // 2, dims[1] - 1):
-            for q in (0, 1, dims[2] // 2, dims[2] - 1):
-                pts.add((a, b, q))
-    return [(i * dims[1] + j) * dims[2] + k for i, j, k in sorted(pts)]
 
 
 # ---- the two retrieval strategies ------------------------------------------------------------
@@ -194,66 +162,40 @@ def plain_halving(comps, V_true, rel, max_iter=8):
             "max_error": err, "max_bound": float(est.max()), "iterations": it + 1}
 
 
-def adjoint_guided(comps, V_true, rel, topk=1 << 16, max_steps=2000):
-    """Greedy over (component, level) planes on the QoI bound itself: the
-    per-level amplifications A_l come from adjoint rows (probe points), the
-    component weighting from the QoI bound composed at the reconstructed
-    field (its sensitivity |V_i| / |V| to first order). Each step adds the
-    plane that lowers the largest QoI bound the most per byte, the maximum
-    taken over the points where the bound is largest."""
-    dims, L = comps[0].dims, comps[0].L
-    dev = V_true[0].device
+def adjoint_guided(comps, V_true, rel, sample_points=4096, max_rounds=12):
+    """Certified QoI planning (see the module docstring): a warm start at
+    4 rel range(V_i) per component with ProDM's L-inf planner (whose extra
+    factor 4 makes it about the certified estimate at rel range(V_i)), then
+    rounds of retrieve, bound over every node, and the QoI greedy planner on
+    the sample_points nodes with the largest bound."""
+    dims, L, nb = comps[0].dims, comps[0].L, comps[0].nb
     Q = torch.sqrt(sum(v.double() ** 2 for v in V_true))
     tau = rel * float(Q.max() - Q.min())
-    A = row_level_norms(dims, L, probe_points(dims), dev).max(dim=0).values.cpu().numpy()
-
-    def E_of(ks):
-        return [sum(float(A[l]) * c.t(l, ks[i][l]) for l in range(L + 1)) for i, c in enumerate(comps)]
-
-    # start from ProDM's first pass (each component to rel * range(V_i)) so
-    # that the bound is evaluated on a meaningful field
+    A = interpret.recompose_amplification(dims, L)
+    tables = [t for c in comps for t in c.tables()]
+    sizes = [[b] * nb for c in comps for b in c.plane_bytes]
     for c, v in zip(comps, V_true):
         c.k = [0] * (L + 1)
-        c.plan_linf(rel * float(v.max() - v.min()))
-    ks = [list(c.k) for c in comps]
+        c.plan_linf(4 * rel * float(v.max() - v.min()))
     rounds = 0
-    for rounds in range(1, 9):
+    for rounds in range(1, max_rounds + 1):
         Vr = [c.retrieve() for c in comps]
-        B = vtot_bound(Vr, E_of(ks))
+        rnd = [c.rounding(A, float(v.abs().max())) for c, v in zip(comps, Vr)]
+        E = [rnd[i] + sum(A[l] * tables[i * (L + 1) + l][c.k[l]] for l in range(L + 1)) for i, c in enumerate(comps)]
+        B = vtot_bound(Vr, E)
         if float(B.max()) <= tau:
             break
-        steps = 0
-        while steps < max_steps:
-            idx = torch.topk(B, min(topk, B.numel())).indices
-            Vs = [v[idx] for v in Vr]
-            cur = float(vtot_bound(Vs, E_of(ks)).max())
-            if cur <= tau:
-                break
-            best, best_gain = None, -1.0
-            for i, c in enumerate(comps):
-                for l in range(L + 1):
-                    if ks[i][l] >= c.nb:
-                        continue
-                    ks[i][l] += 1
-                    val = float(vtot_bound(Vs, E_of(ks)).max())
-                    ks[i][l] -= 1
-                    gain = (cur - val) / c.plane_bytes[l]
-                    if gain > best_gain:
-                        best, best_gain = (i, l), gain
-            if best is None:
-                break
-            ks[best[0]][best[1]] += 1
-            steps += 1
-            B = vtot_bound(Vr, E_of(ks))
+        top = torch.topk(B, min(sample_points, B.numel())).indices
+        sample = torch.stack([v[top].double() for v in Vr], dim=1).cpu().numpy()
+        idx = np.array([k for c in comps for k in c.k], np.uint8)
+        _, idx = interpret.qoi_retrieve_sizes(sizes, tables, A, sample, tau, idx, rounding=rnd)
         for i, c in enumerate(comps):
-            c.k = list(ks[i])
-    Vr = [c.retrieve() for c in comps]
-    est = vtot_bound(Vr, E_of(ks))
+            c.k = [int(v) for v in idx[i * (L + 1):(i + 1) * (L + 1)]]
     Qr = torch.sqrt(sum(v.double() ** 2 for v in Vr))
     err = float(torch.max(torch.abs(Qr - Q)))
     return {"bytes": sum(c.bytes() for c in comps), "planes": [list(c.k) for c in comps], "tau": tau,
-            "max_error": err, "max_bound": float(est.max()), "rounds": rounds,
-            "amplification": [round(float(a), 4) for a in A]}
+            "max_error": err, "max_bound": float(B.max()), "rounds": rounds,
+            "amplification": [round(float(a), 6) for a in A]}
 
 
 def run(n=257, L=5, nb=32, rel=1e-3, device="cuda"):
