@@ -1,6 +1,6 @@
 // composed.hpp -- composers with the template shapes of ProDM's
 // ComposedRefactor (Refactor/ComposedRefactor.hpp:15-141) and
-// ComposedReconstructor (Reconstructor/ComposedReconstructor.hpp:17-300).
+// ComposedReconstructor (Reconstructor/ComposedReconstructor.hpp:17-297).
 //
 // They drive any components with ProDM's interfaces (the GPU ones of
 // components.hpp, or ProDM's CPU ones) in ProDM's order and write / read

@@ -13,8 +13,8 @@
 //
 // Planes are compressed in parallel on host threads. Each plane's bytes are
 // what the sequential ProDM compressor writes (ZSTD is deterministic per
-// input and level); the adaptive compressor compresses every plane and
-// keeps the sequential decision. The destination capacity is
+// input and level), and the adaptive compressor makes ProDM's sequential
+// decision. The destination capacity is
 // ZSTD_compressBound (ProDM sizes it by 1.2x the input, ZSTD.hpp:12-16).
 #ifndef PRODM_COMPRESSORS_HPP
 #define PRODM_COMPRESSORS_HPP
@@ -129,19 +129,38 @@ class AdaptiveLevelCompressor {
         : latter_(latter), threads_(threads) {}
     AdaptiveLevelCompressor(const AdaptiveLevelCompressor &o) : latter_(o.latter_), threads_(o.threads_) {}
     ~AdaptiveLevelCompressor() { decompress_release(); }
+    // ProDM compresses planes in order until the first plane i >= 1 whose
+    // ratio is below 1.05 (the stopping index), then the planes from
+    // max(latter, stop + 1) on. So every plane at or above `latter` ends up
+    // compressed whatever the stop, and a plane below it only when it is at
+    // or before the stop. Host threads take planes in index order; a plane
+    // below `latter` that lies past a stop already found is skipped, so the
+    // only compression discarded is that of planes already in flight when
+    // the stop is found (fewer than the thread count). The stop is the first
+    // low-ratio plane: every plane before it is compressed.
     uint8_t compress_level(std::vector<uint8_t *> &streams, std::vector<uint32_t> &sizes) const {
         const int nb = (int)streams.size();
         std::vector<uint8_t *> c(nb, nullptr);
         std::vector<uint32_t> cs(nb, 0);
-        parallel_for(nb, threads_, [&](int i) { cs[i] = zstd::compress(streams[i], sizes[i], &c[i]); });
-        int stop = nb;
-        for (int i = 1; i < nb; i++) {
+        std::vector<char> low(nb, 0);
+        std::atomic<int> stop_seen(nb);
+        parallel_for(nb, threads_, [&](int i) {
+            if (i < latter_ && i > stop_seen.load()) return;
+            cs[i] = zstd::compress(streams[i], sizes[i], &c[i]);
             const float ratio = (float)(sizes[i] * 1.0 / cs[i]);  // a float, compared in double
-            if (ratio < 1.05) {
+            if (i >= 1 && ratio < 1.05) {
+                low[i] = 1;
+                int cur = stop_seen.load();
+                while (i < cur && !stop_seen.compare_exchange_weak(cur, i)) {
+                }
+            }
+        });
+        int stop = nb;
+        for (int i = 1; i < nb; i++)
+            if (low[i]) {
                 stop = i;
                 break;
             }
-        }
         const int latter_start = stop < latter_ ? latter_ : stop + 1;
         for (int i = 0; i < nb; i++) {
             if (i <= stop || i >= latter_start) {
@@ -149,7 +168,7 @@ class AdaptiveLevelCompressor {
                 streams[i] = c[i];
                 sizes[i] = cs[i];
             } else {
-                std::free(c[i]);
+                std::free(c[i]);  // in flight past the stop (or never compressed: null)
             }
         }
         return (uint8_t)stop;

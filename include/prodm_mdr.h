@@ -211,7 +211,7 @@ int prodm_grouped_level_errors_f32(const float *d_x, uint64_t n, int32_t exp, in
 /* ---- device-resident refactor / retrieve pipeline: the GPU stages of
  *      ComposedRefactor::refactor (ComposedRefactor.hpp:78-141: decompose,
  *      interleave, max-abs, encode) and of ComposedReconstructor::reconstruct
- *      (ComposedReconstructor.hpp:349-420: decode, reposition, recompose).
+ *      (ComposedReconstructor.hpp:187-258: decode, reposition, recompose).
  *      Interleave is fused into the decomposition (coefficients are written
  *      straight into per-level buffers) and reposition into the
  *      recomposition. Lossless compression and file I/O stay on the host.

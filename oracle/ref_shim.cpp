@@ -3,7 +3,8 @@
 // oracle/_ref/libref.so. TEST INFRASTRUCTURE ONLY: it is the compiled
 // reference used to pin the oracle and to write golden fixtures; nothing in
 // the product links it. Only the headers that build without MGARDx are used
-// (encoders, collectors, interleaver, estimators, interpreters, compressors).
+// (encoders, collectors, interleaver, estimators, interpreters, compressors,
+// serialize/deserialize, the concatenated-level file writer and retriever).
 #include <algorithm>
 #include <cassert>
 #include <cmath>
@@ -27,6 +28,8 @@
 #include "MDR/SizeInterpreter/GreedyBasedSizeInterpreter.hpp"
 #include "MDR/SizeInterpreter/BasicSizeInterpreter.hpp"
 #include "MDR/LosslessCompressor/LevelCompressor.hpp"
+#include "MDR/Retriever/FileRetriever.hpp"
+#include "MDR/Writer/FileWriter.hpp"
 
 namespace {
 // the reference interpreters print progress to std::cout; keep it quiet
@@ -329,4 +332,154 @@ int ref_compress_level(int kind, int latter, int nb, const uint8_t *in, const ui
     return rc;
 }
 
+}  // extern "C"
+
+// ---- ProDM's file layout, read and written by ProDM's own code ------------
+namespace {
+template <class T>
+int read_mdr(const char *metadata_path, const char *const *level_paths, int nfiles, int latter, const uint8_t *k1,
+             uint32_t *dims, int *ndims, int *nlevels, double *bounds, int *nplanes, uint32_t *level_sizes_out,
+             uint8_t *stopping, uint32_t *level_num_out, int *negabinary, uint8_t *out, uint64_t cap,
+             uint64_t *raw_sizes, uint64_t *retrieved) {
+    std::vector<std::string> files(level_paths, level_paths + nfiles);
+    MDR::ConcatLevelFileRetriever retriever(metadata_path, files);
+    // the field order of ComposedReconstructor::load_metadata (ComposedReconstructor.hpp:133-145)
+    uint8_t *metadata = retriever.load_metadata();
+    uint8_t const *pos = metadata;
+    std::vector<uint32_t> dimensions;
+    std::vector<T> level_error_bounds;
+    std::vector<std::vector<uint32_t>> level_sizes;
+    std::vector<uint8_t> stopping_indices;
+    std::vector<uint32_t> level_num;
+    uint8_t num_dims = *(pos++);
+    MDR::deserialize(pos, num_dims, dimensions);
+    uint8_t num_levels = *(pos++);
+    MDR::deserialize(pos, num_levels, level_error_bounds);
+    MDR::deserialize(pos, num_levels, level_sizes);
+    MDR::deserialize(pos, num_levels, stopping_indices);
+    MDR::deserialize(pos, num_levels, level_num);
+    const int nega = *(pos++);
+    free(metadata);
+    if (num_levels != nfiles || num_dims > 3) return -1;
+    *ndims = num_dims;
+    *nlevels = num_levels;
+    *negabinary = nega;
+    for (int d = 0; d < num_dims; d++) dims[d] = dimensions[d];
+    const int nb = level_sizes.empty() ? 0 : (int)level_sizes[0].size();
+    *nplanes = nb;
+    for (int l = 0; l < num_levels; l++) {
+        if ((int)level_sizes[l].size() != nb) return -2;
+        bounds[l] = (double)level_error_bounds[l];
+        stopping[l] = stopping_indices[l];
+        level_num_out[l] = level_num[l];
+        for (int p = 0; p < nb; p++) level_sizes_out[l * nb + p] = level_sizes[l][p];
+    }
+    // two progressive steps through the retriever's per-level offsets, as a
+    // reconstructor reads: planes [0, k1[l]), then the rest
+    std::vector<std::vector<std::vector<uint8_t>>> planes(num_levels, std::vector<std::vector<uint8_t>>(nb));
+    std::vector<uint8_t> prev(num_levels, 0), held(num_levels);
+    for (int step = 0; step < 2; step++) {
+        std::vector<uint32_t> rs(num_levels, 0);
+        for (int l = 0; l < num_levels; l++) {
+            held[l] = step == 0 ? std::min<int>(k1[l], nb) : nb;
+            for (int p = prev[l]; p < held[l]; p++) rs[l] += level_sizes[l][p];
+        }
+        auto comps = retriever.retrieve_level_components(level_sizes, rs, prev, held);
+        for (int l = 0; l < num_levels; l++) {
+            const int k = held[l] - prev[l];
+            if (k <= 0) continue;
+            std::vector<uint64_t> raw(k);
+            for (int i = 0; i < k; i++) {  // decompressed length: the frame's size_t prefix
+                const int p = prev[l] + i;
+                size_t len = level_sizes[l][p];
+                if (p <= stopping_indices[l] || p >= latter) memcpy(&len, comps[l][i], sizeof(size_t));
+                raw[i] = len;
+            }
+            MDR::AdaptiveLevelCompressor c(latter);
+            c.decompress_level(comps[l], level_sizes[l], prev[l], (uint8_t)k, stopping_indices[l]);
+            for (int i = 0; i < k; i++) planes[l][prev[l] + i].assign(comps[l][i], comps[l][i] + raw[i]);
+            c.decompress_release();
+        }
+        prev = held;
+    }
+    *retrieved = retriever.get_retrieved_size();
+    retriever.release();
+    uint64_t off = 0;
+    for (int l = 0; l < num_levels; l++)
+        for (int p = 0; p < nb; p++) {
+            const auto &v = planes[l][p];
+            raw_sizes[l * nb + p] = v.size();
+            if (off + v.size() > cap) return -3;
+            if (!v.empty()) memcpy(out + off, v.data(), v.size());
+            off += v.size();
+        }
+    return 0;
+}
+
+// streams: level-major, plane order, back to back, raw sizes in sizes
+template <class T>
+int write_mdr(const char *metadata_path, const char *const *level_paths, int ndims, const uint32_t *dims,
+              int nlevels, int nb, const double *bounds, int negabinary, int latter, const uint8_t *streams,
+              const uint32_t *sizes) {
+    std::vector<uint32_t> dimensions(dims, dims + ndims);
+    std::vector<T> level_error_bounds;
+    std::vector<std::vector<uint8_t *>> level_components(nlevels);
+    std::vector<std::vector<uint32_t>> level_sizes(nlevels);
+    std::vector<uint8_t> stopping_indices;
+    const uint8_t *p = streams;
+    for (int l = 0; l < nlevels; l++) {
+        level_error_bounds.push_back((T)bounds[l]);
+        for (int q = 0; q < nb; q++) {
+            const uint32_t n = sizes[l * nb + q];
+            uint8_t *b = (uint8_t *)malloc(n ? n : 1);
+            memcpy(b, p, n);
+            p += n;
+            level_components[l].push_back(b);
+            level_sizes[l].push_back(n);
+        }
+        stopping_indices.push_back(MDR::AdaptiveLevelCompressor(latter).compress_level(level_components[l], level_sizes[l]));
+    }
+    MDR::ConcatLevelFileWriter writer(metadata_path, std::vector<std::string>(level_paths, level_paths + nlevels));
+    std::vector<uint32_t> level_num = writer.write_level_components(level_components, level_sizes);
+    // the field order of ComposedRefactor::write_metadata (ComposedRefactor.hpp:48-67)
+    using MDR::get_size;
+    const uint32_t metadata_size = sizeof(uint8_t) + get_size(dimensions) + sizeof(uint8_t) +
+                                   get_size(level_error_bounds) + get_size(level_sizes) +
+                                   get_size(stopping_indices) + get_size(level_num) + 1;
+    uint8_t *metadata = (uint8_t *)malloc(metadata_size);
+    uint8_t *pos = metadata;
+    *(pos++) = (uint8_t)dimensions.size();
+    MDR::serialize(dimensions, pos);
+    *(pos++) = (uint8_t)level_error_bounds.size();
+    MDR::serialize(level_error_bounds, pos);
+    MDR::serialize(level_sizes, pos);
+    MDR::serialize(stopping_indices, pos);
+    MDR::serialize(level_num, pos);
+    *(pos++) = (uint8_t)negabinary;
+    writer.write_metadata(metadata, metadata_size);
+    free(metadata);
+    for (auto &lv : level_components)
+        for (uint8_t *b : lv) free(b);
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+int ref_read_mdr(const char *metadata_path, const char *const *level_paths, int nfiles, int dtype, int latter,
+                 const uint8_t *k1, uint32_t *dims, int *ndims, int *nlevels, double *bounds, int *nplanes,
+                 uint32_t *level_sizes, uint8_t *stopping, uint32_t *level_num, int *negabinary, uint8_t *out,
+                 uint64_t cap, uint64_t *raw_sizes, uint64_t *retrieved) {
+    return dtype ? read_mdr<double>(metadata_path, level_paths, nfiles, latter, k1, dims, ndims, nlevels, bounds,
+                                    nplanes, level_sizes, stopping, level_num, negabinary, out, cap, raw_sizes, retrieved)
+                 : read_mdr<float>(metadata_path, level_paths, nfiles, latter, k1, dims, ndims, nlevels, bounds,
+                                   nplanes, level_sizes, stopping, level_num, negabinary, out, cap, raw_sizes, retrieved);
+}
+int ref_write_mdr(const char *metadata_path, const char *const *level_paths, int ndims, const uint32_t *dims,
+                  int nlevels, int nb, int dtype, const double *bounds, int negabinary, int latter,
+                  const uint8_t *streams, const uint32_t *sizes) {
+    return dtype ? write_mdr<double>(metadata_path, level_paths, ndims, dims, nlevels, nb, bounds, negabinary, latter,
+                                     streams, sizes)
+                 : write_mdr<float>(metadata_path, level_paths, ndims, dims, nlevels, nb, bounds, negabinary, latter,
+                                    streams, sizes);
+}
 }  // extern "C"

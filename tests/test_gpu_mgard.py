@@ -43,7 +43,7 @@ def test_decompose_matches_oracle(gpu, dims, L, dt):
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
 def test_strided_box(gpu, dt):
     """decompose a sub-box of a larger array (ComposedReconstructor passes
-    full-array strides with a reduced box, ComposedReconstructor.hpp:412-415)."""
+    full-array strides with a reduced box, ComposedReconstructor.hpp:250-253)."""
     from prodm_amd import mdr
     full = [33, 40, 70]
     box = [17, 33, 65]
