@@ -138,6 +138,20 @@ int prodm_grouped_encode_f32(const float *d_x, uint32_t n, int32_t exp, int num_
 int prodm_grouped_encode_f64(const double *d_x, uint32_t n, int32_t exp, int num_bitplanes, int compat, uint8_t *d_rb,
                              uint64_t *d_planes, uint64_t plane_stride_words, uint32_t *d_words, void *d_ws,
                              size_t ws_bytes, prodm_stream_t stream);
+/* Allocation-free variant of prodm_grouped_encode_f32 (same arguments, same
+ * results, no workspace). No counterpart in ProDM: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc). The encode's per-workgroup
+ * histograms and plane offsets live in an 8 MiB device array of the library
+ * that each GPU holds until the process ends (its address is looked up once
+ * per device on the first call); n elements with nb planes need
+ * prodm_grouped_workspace_bytes(n, nb, PRODM_F32) bytes of it (about 3.8 MB
+ * at 118 M elements and 32 planes; the limit is about 264 M elements at 32
+ * planes), larger requests return PRODM_ERR_UNSUPPORTED. Calls that overlap on one GPU share the array and
+ * race: serialize them.                                                   */
+int prodm_grouped_encode_noalloc_f32(const float *d_x, uint32_t n, int32_t exp, int num_bitplanes, int compat,
+                                     uint8_t *d_rb, uint32_t *d_planes, uint64_t plane_stride_words, uint32_t *d_words,
+                                     prodm_stream_t stream);
 typedef struct prodm_grouped_state prodm_grouped_state;
 int prodm_grouped_state_create(prodm_grouped_state **out, int max_levels, int dtype);  /* max_levels 1..64 */
 int prodm_grouped_state_destroy(prodm_grouped_state *state);

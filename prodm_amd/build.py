@@ -16,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libprodm_mdr.so")
-SOURCES = ["capi.hip", "mgard.hip", "bitplane.hip", "interleave.hip", "collect.hip", "adjoint.hip", "batched.hip"]
+SOURCES = ["capi.hip", "mgard.hip", "bitplane.hip", "interleave.hip", "collect.hip", "adjoint.hip", "batched.hip",
+           "noalloc.hip"]
 HOST_SOURCES = ["planner.cpp", "perbit_host.cpp"]  # host-only C++ (g++)
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 

@@ -51,6 +51,7 @@ _SIGS = {
     "prodm_grouped_workspace_bytes": (sz, [u32, C.c_int, C.c_int]),
     "prodm_grouped_encode_f32": (C.c_int, [vp, u32, i32, C.c_int, C.c_int, vp, vp, u64, vp, vp, sz, vp]),
     "prodm_grouped_encode_f64": (C.c_int, [vp, u32, i32, C.c_int, C.c_int, vp, vp, u64, vp, vp, sz, vp]),
+    "prodm_grouped_encode_noalloc_f32": (C.c_int, [vp, u32, i32, C.c_int, C.c_int, vp, vp, u64, vp, vp]),
     "prodm_grouped_state_create": (C.c_int, [vp, C.c_int, C.c_int]),
     "prodm_grouped_state_destroy": (C.c_int, [vp]),
     "prodm_grouped_decode_f32": (C.c_int, [vp, C.c_int, vp, u32, i32, C.c_int, C.c_int, vp, vp]),
