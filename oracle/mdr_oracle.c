@@ -166,16 +166,19 @@ static void solve3(double *f, const uint32_t c[3]) {
     double *inv = (double *)malloc(sizeof(double) * mx), *cp = (double *)malloc(sizeof(double) * mx);
     if (c[2] > 1) {
         thomas_factors(c[2], inv, cp);
+#pragma omp parallel for schedule(static)
         for (uint64_t l = 0; l < (uint64_t)c[0] * c[1]; l++) thomas_line(f + l * c[2], 1, c[2], inv, cp);
     }
     if (c[1] > 1) {
         thomas_factors(c[1], inv, cp);
+#pragma omp parallel for schedule(static)
         for (uint32_t a = 0; a < c[0]; a++)
             for (uint32_t b = 0; b < c[2]; b++)
                 thomas_line(f + (uint64_t)a * c[1] * c[2] + b, c[2], c[1], inv, cp);
     }
     if (c[0] > 1) {
         thomas_factors(c[0], inv, cp);
+#pragma omp parallel for schedule(static)
         for (uint64_t l = 0; l < (uint64_t)c[1] * c[2]; l++)
             thomas_line(f + l, (uint64_t)c[1] * c[2], c[0], inv, cp);
     }

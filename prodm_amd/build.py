@@ -84,6 +84,24 @@ def build_host(jobs=None):
     os.replace(HOST_LIB + ".tmp", HOST_LIB)
 
 
+SAN_DIR = os.path.join(BUILD, "san")
+
+
+def build_host_sanitized():
+    """libprodm_host.so under AddressSanitizer + UndefinedBehaviorSanitizer
+    (host code, g++) in prodm_amd/_build/san/, for a CPU test run with
+    PRODM_HOST_LIB pointing at it and gcc's libasan preloaded."""
+    os.makedirs(SAN_DIR, exist_ok=True)
+    out = os.path.join(SAN_DIR, "libprodm_host.so")
+    cmd = [os.environ.get("CXX", "g++"), "-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+           "-fno-sanitize-recover=undefined", "-std=c++17", "-Wall", "-shared", "-fPIC", f"-I{INCLUDE}",
+           "-idirafter", ZSTD_INC, os.path.join(CSRC, "host_capi.cpp"), "-o", out, "-l:libzstd.so.1", "-pthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"sanitizer build failed:\n{r.stderr}")
+    return out
+
+
 def build(jobs=None, verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     srcs = SOURCES + HOST_SOURCES
@@ -108,5 +126,6 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=None)
     ap.add_argument("-v", "--verbose", action="store_true")
+    ap.add_argument("--sanitize", action="store_true", help="only the sanitizer build of the host library")
     a = ap.parse_args()
-    print(build(a.jobs, a.verbose))
+    print(build_host_sanitized() if a.sanitize else build(a.jobs, a.verbose))

@@ -274,6 +274,7 @@ static void plan_free(prodm_mdr_plan *p) {
         if (e) (void)hipEventDestroy(e);
     if (p->join_ev) (void)hipEventDestroy(p->join_ev);
     if (p->side) (void)hipStreamDestroy(p->side);
+    if (p->side2) (void)hipStreamDestroy(p->side2);
     void *ptrs[] = {p->ws, p->levels, p->keys, p->planes, p->rb, p->words, p->signs, p->gbws, p->flev};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
@@ -282,9 +283,22 @@ static void plan_free(prodm_mdr_plan *p) {
 static int plan_side(prodm_mdr_plan *p) {
     if (p->side) return kOk;
     PRODM_HIP(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
+    PRODM_HIP(hipStreamCreateWithFlags(&p->side2, hipStreamNonBlocking));
     for (int l = 0; l <= p->g.L; l++) PRODM_HIP(hipEventCreateWithFlags(&p->lev_ev[l], hipEventDisableTiming));
     PRODM_HIP(hipEventCreateWithFlags(&p->join_ev, hipEventDisableTiming));
     return kOk;
+}
+
+// per-level load vectors of the retrieve pipeline (allocated on first use,
+// before any graph capture)
+static int plan_flev(prodm_mdr_plan *p) {
+    if (p->flev || p->hier || p->g.L < 1) return kOk;
+    uint64_t total = 0;
+    for (int l = 1; l <= p->g.L; l++) {
+        p->flev_off[l] = total;
+        total += (mgard_level_f_bytes(p->g, l) + 255) & ~(uint64_t)255;
+    }
+    return plan_alloc(&p->flev, total, p->device_bytes);
 }
 
 template <class F> static int hook_call(void *ctx, int level) { return (*static_cast<F *>(ctx))(level); }
@@ -317,8 +331,9 @@ template <class T> static int plan_refactor(prodm_mdr_plan *p, const T *in, hipS
     };
     // per-stage timing keeps every launch on one stream; otherwise each
     // level's encode runs on the side stream as soon as its coefficients and
-    // max are final, beside the decomposition of the coarser levels
-    if (tm || p->capturing) {
+    // max are final, beside the decomposition of the coarser levels (also
+    // under graph capture: the fork and join become graph edges)
+    if (tm) {
         rc = mgard_decompose<T>(in, st, g, sink, keys, p->ws, p->ws_bytes, s, tm, true, p->hier != 0);
         for (int l = g.L; l >= 0 && !rc; l--) rc = encode_level(l, s);
         return rc;
@@ -421,7 +436,7 @@ template <class T> static int plan_retrieve(prodm_mdr_plan *p, const uint8_t *k,
     };
     const uint64_t st[3] = {(uint64_t)g.dims[1] * g.dims[2], g.dims[2], 1};
     int rc = kOk;
-    if (tm || p->capturing) {
+    if (tm) {
         for (int l = g.L; l >= 0 && !rc; l--) rc = decode_level(l, s);
         if (rc) return rc;
         return mgard_recompose<T>(src, g, 1, out, st, p->ws, p->ws_bytes, s, tm, p->hier != 0);
@@ -429,31 +444,33 @@ template <class T> static int plan_retrieve(prodm_mdr_plan *p, const uint8_t *k,
     // on the side stream, coarsest first: each level's decode and, for the
     // orthogonal decomposer, its recompose load sweep into a per-level load
     // vector (it reads only that level's coefficients); the main stream's
-    // chain of solves and interpolations waits for level l just before it
+    // chain of solves and interpolations waits for level l just before it.
+    // The finest level's decode and load sweep, the bulk of the bytes, start
+    // at once on a second side stream instead of after the coarser levels'
+    // short, latency-bound launches (negabinary: its decode needs no
+    // workspace, so the two side streams share nothing).
     if ((rc = plan_side(p))) return rc;
     const bool ahead = p->hier == 0 && g.L >= 1;
     double *fl[33] = {nullptr};
     if (ahead) {
-        if (!p->flev) {
-            uint64_t total = 0;
-            for (int l = 1; l <= g.L; l++) {
-                p->flev_off[l] = total;
-                total += (mgard_level_f_bytes(g, l) + 255) & ~(uint64_t)255;
-            }
-            if ((rc = plan_alloc(&p->flev, total, p->device_bytes))) return rc;
-        }
+        if ((rc = plan_flev(p))) return rc;
         for (int l = 1; l <= g.L; l++) fl[l] = reinterpret_cast<double *>(static_cast<char *>(p->flev) + p->flev_off[l]);
     }
     PRODM_HIP(hipEventRecord(p->join_ev, s));
     PRODM_HIP(hipStreamWaitEvent(p->side, p->join_ev, 0));
-    for (int l = 0; l <= g.L && !rc; l++) {
-        rc = decode_level(l, p->side);
-        if (!rc && ahead && l >= 1) rc = mgard_recompose_load<T>(src, g, l, fl[l], p->side);
-        if (!rc && hipEventRecord(p->lev_ev[l], p->side) != hipSuccess) { set_error("retrieve: event record"); rc = kErrHip; }
+    const bool split = p->encoder == PRODM_ENC_NEGABINARY && g.L >= 1;
+    if (split) PRODM_HIP(hipStreamWaitEvent(p->side2, p->join_ev, 0));
+    for (int i = 0; i <= g.L && !rc; i++) {
+        const int l = split ? (i == 0 ? g.L : i - 1) : i;
+        hipStream_t ss = (split && l == g.L) ? p->side2 : p->side;
+        rc = decode_level(l, ss);
+        if (!rc && ahead && l >= 1) rc = mgard_recompose_load<T>(src, g, l, fl[l], ss);
+        if (!rc && hipEventRecord(p->lev_ev[l], ss) != hipSuccess) { set_error("retrieve: event record"); rc = kErrHip; }
     }
     if (rc) {
         // join whatever was forked before reporting the error
         if (hipEventRecord(p->join_ev, p->side) == hipSuccess) (void)hipStreamWaitEvent(s, p->join_ev, 0);
+        if (split && hipEventRecord(p->join_ev, p->side2) == hipSuccess) (void)hipStreamWaitEvent(s, p->join_ev, 0);
         return rc;
     }
     auto need = [&](int lv) -> int {
@@ -943,9 +960,10 @@ int prodm_mdr_refactor_graph(prodm_mdr_plan *p, const void *d_in, prodm_stream_t
     if (!p || !d_in) { set_error("refactor_graph: null argument"); return kErrArg; }
     hipStream_t s = (hipStream_t)stream;
     if (!p->g_ref || p->g_ref_in != d_in) {
+        if (int r = prodm::plan_side(p)) return r;  // streams and events exist before the capture
         const bool on = p->timer.on;
-        p->timer.on = false;  // no event nodes in the graph
-        p->capturing = true;  // one stream: a forked graph replays slower than the eager fork
+        p->timer.on = false;  // no timing events in the graph
+        p->capturing = true;
         const int rc = prodm::graph_capture(s, &p->g_ref, [&] { return prodm_mdr_refactor(p, d_in, stream); });
         p->capturing = false;
         p->timer.on = on;
@@ -964,6 +982,8 @@ int prodm_mdr_retrieve_graph(prodm_mdr_plan *p, const uint8_t *k, void *d_out, p
     }
     hipStream_t s = (hipStream_t)stream;
     if (!p->g_ret || p->g_ret_out != d_out || std::memcmp(p->g_ret_k, k, p->g.L + 1) != 0) {
+        if (int r = prodm::plan_side(p)) return r;
+        if (int r = prodm::plan_flev(p)) return r;
         const bool on = p->timer.on;
         p->timer.on = false;
         p->capturing = true;

@@ -37,9 +37,10 @@ struct prodm_mdr_plan {
     // coarser ones, and the decode of a level the recomposition of the finer
     // ones (events per level, created on first use)
     hipStream_t side = nullptr;
+    hipStream_t side2 = nullptr;  // retrieve: the finest level's decode and load sweep
     hipEvent_t lev_ev[33] = {};
     hipEvent_t join_ev = nullptr;
-    bool capturing = false;  // graph capture records the single-stream order
+    bool capturing = false;  // a graph capture is in progress (the PerBit retrieve has no graph form)
     // per-level load vectors of the retrieve pipeline (allocated on first use)
     void *flev = nullptr;
     uint64_t flev_off[33] = {0};

@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+# PRODM_ORACLE_LIB selects another build of the same oracle (e.g. the sanitizer build)
+ORACLE_SO = os.environ.get("PRODM_ORACLE_LIB") or os.path.join(ROOT, "oracle", "liboracle.so")
+ORACLE_MT_SO = os.path.join(ROOT, "oracle", "liboracle_mt.so")  # OpenMP build (all-core CPU baseline)
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libref.so")
 
 u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
@@ -30,6 +32,22 @@ def oracle():
             raise RuntimeError("oracle not built: run `make -C oracle`")
         _oracle = C.CDLL(ORACLE_SO)
     return _oracle
+
+
+def use_library(path):
+    """Point the bindings at another build of the oracle (ORACLE_MT_SO: the
+    OpenMP build, same results); returns the previous library object."""
+    global _oracle
+    prev = _oracle
+    if not os.path.exists(path):
+        raise RuntimeError(f"oracle build not found: {path}")
+    _oracle = C.CDLL(path)
+    return prev
+
+
+def restore_library(prev):
+    global _oracle
+    _oracle = prev
 
 
 def ref_available():

@@ -10,7 +10,7 @@ import pytest
 
 import oracle_bind as ob
 
-HOST = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "prodm_amd", "libprodm_host.so")
+HOST = os.environ.get("PRODM_HOST_LIB") or os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "prodm_amd", "libprodm_host.so")
 
 
 @pytest.fixture(scope="module")

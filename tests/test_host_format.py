@@ -28,7 +28,7 @@ import oracle_bind as ob
 from fields import bumps_field
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HOST = os.path.join(ROOT, "prodm_amd", "libprodm_host.so")
+HOST = os.environ.get("PRODM_HOST_LIB") or os.path.join(ROOT, "prodm_amd", "libprodm_host.so")
 
 pytestmark = pytest.mark.skipif(not ob.ref_available(), reason="compiled reference not built (oracle/_ref)")
 
