@@ -232,7 +232,10 @@ int prodm_grouped_level_errors_f32(const float *d_x, uint64_t n, int32_t exp, in
  *      encoder: PRODM_ENC_NEGABINARY, PRODM_ENC_GROUPED or PRODM_ENC_PERBIT
  *      (PerBitBPEncoder: encoded on the device; prodm_mdr_retrieve decodes
  *      its streams on the host, see prodm_perbit_decode_f32, and has no
- *      graph form).                                                         */
+ *      graph form). A plane of the last two dims must span less than 2 GiB
+ *      and a level fewer than 2^32 elements: plan creation refuses larger
+ *      geometries (PRODM_ERR_UNSUPPORTED / PRODM_ERR_ARG) before any device
+ *      call. Lines of any length are solved.                                 */
 typedef struct prodm_mdr_plan prodm_mdr_plan;
 int prodm_mdr_plan_create(prodm_mdr_plan **out, const uint32_t *dims, int ndims, int target_level, int num_bitplanes,
                           int dtype, int encoder);

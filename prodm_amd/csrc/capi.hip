@@ -85,10 +85,17 @@ static void strides3(const LevelGeom &g, const uint32_t *strides, uint64_t *s) {
         for (int d = 0; d < g.nd - 1; d++) s[3 - g.nd + d] = strides[d];
 }
 
-static int check_geom(const LevelGeom &g) {
+// elem > 0 also checks the plane limit of the level sweeps: they address a
+// fine plane (the last two dims) with 31-bit byte offsets
+static int check_geom(const LevelGeom &g, size_t elem) {
     if (!g.ok) {
         set_error("unsupported geometry: every dim > 1 must be odd (>= 3) at every level above 0, ndims 1..3");
         return kErrGeometry;
+    }
+    if (elem && (uint64_t)g.ld[g.L][1] * g.ld[g.L][2] * elem >= 0x7FFFFFFFull) {
+        set_error("unsupported geometry: a plane of the last two dims spans 2 GiB or more; the level sweeps address "
+                  "a plane with 31-bit offsets");
+        return kErrUnsupported;
     }
     return kOk;
 }
@@ -97,7 +104,7 @@ template <class T>
 static int decompose_impl(const T *in, T *out, const uint32_t *dims, int nd, int L, const uint32_t *strides, void *ws,
                           size_t ws_bytes, hipStream_t s, bool hier = false) {
     const LevelGeom g = make_geom(dims, nd, L);
-    if (int rc = check_geom(g)) return rc;
+    if (int rc = check_geom(g, sizeof(T))) return rc;
     if (!in || !out || in == out) { set_error("decompose: need distinct device input and output"); return kErrArg; }
     uint64_t st[3];
     strides3(g, strides, st);
@@ -111,7 +118,7 @@ template <class T>
 static int recompose_impl(const T *in, T *out, const uint32_t *dims, int nd, int L, const uint32_t *strides, void *ws,
                           size_t ws_bytes, hipStream_t s, bool hier = false) {
     const LevelGeom g = make_geom(dims, nd, L);
-    if (int rc = check_geom(g)) return rc;
+    if (int rc = check_geom(g, sizeof(T))) return rc;
     if (!in || !out || in == out) { set_error("recompose: need distinct device input and output"); return kErrArg; }
     uint64_t st[3];
     strides3(g, strides, st);
@@ -125,7 +132,7 @@ template <class T>
 static int adjoint_impl(const T *g_in, T *out, const uint32_t *dims, int nd, int L, const uint32_t *strides, void *ws,
                         size_t ws_bytes, hipStream_t s) {
     const LevelGeom g = make_geom(dims, nd, L);
-    if (int rc = check_geom(g)) return rc;
+    if (int rc = check_geom(g, sizeof(T))) return rc;
     if (!g_in || !out || (const void *)g_in == (const void *)out) {
         set_error("recompose_adjoint: need distinct device input and output");
         return kErrArg;
@@ -718,7 +725,7 @@ int prodm_mdr_plan_create(prodm_mdr_plan **out, const uint32_t *dims, int ndims,
     }
     prodm_mdr_plan *p = new prodm_mdr_plan();
     p->g = make_geom(dims, ndims, target_level);
-    if (int rc = check_geom(p->g)) { delete p; return rc; }
+    if (int rc = check_geom(p->g, dtype == PRODM_F64 ? 8 : 4)) { delete p; return rc; }
     for (int l = 0; l <= target_level; l++)
         if (p->g.elems[l] > 0xffffffffull) { delete p; set_error("plan: level larger than 2^32 elements"); return kErrArg; }
     p->dtype = dtype;

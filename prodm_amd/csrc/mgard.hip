@@ -753,6 +753,73 @@ __global__ __launch_bounds__(512) void k_solve_strided(SolveParams<T> S, uint32_
     solve_strided_lines<T, SEG>(S, m, nlines, d, blockDim.x / 64, blockIdx.x);
 }
 
+// Lines longer than the register-resident solves hold (dim 2: more than
+// 64 x 16 nodes; dims 1 / 0: more than 8 x 72): a lane per line runs the
+// Thomas recurrences in node order, the forward sweep writing y in place
+// and the backward sweep reading it back (two passes over the line instead
+// of one). Lines of dims 1 / 0 are indexed as in the strided solve, so a
+// wave's accesses are one 512-byte run; dim-2 lanes sit a row apart.
+template <class T>
+__global__ __launch_bounds__(256) void k_solve_seq(SolveParams<T> S, uint32_t m, uint64_t nlines, int d) {
+    const uint64_t q = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= nlines) return;
+    const uint64_t fp = S.fp, c1 = S.c[1], c2 = S.c[2];
+    uint64_t base, stride, ubase, ustride;
+    if (d == 2) {
+        base = q * fp; stride = 1;
+        ubase = q * c2; ustride = 1;
+    } else {
+        const uint64_t j2 = q % fp, r = q / fp;
+        if (j2 >= c2) return;
+        if (d == 1) {
+            base = r * c1 * fp + j2; stride = fp;
+            ubase = r * c1 * c2 + j2; ustride = c2;
+        } else {
+            base = r * fp + j2; stride = c1 * fp;
+            ubase = r * c2 + j2; ustride = c1 * c2;
+        }
+    }
+    PRODM_GLOBAL double *f = (PRODM_GLOBAL double *)S.f;
+    const PRODM_CONST double *inv = (const PRODM_CONST double *)S.inv;
+    constexpr uint32_t U = 8;  // loads issued ahead of the dependent chain
+    double y = 0.0;
+    uint32_t i = 0;
+    for (; i + U <= m; i += U) {
+        double t[U];
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) t[k] = f[base + (i + k) * stride];
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) {
+            y = (t[k] - y) * thomas_factor(S, inv, m, i + k);
+            f[base + (i + k) * stride] = y;
+        }
+    }
+    for (; i < m; i++) {
+        y = (f[base + i * stride] - y) * thomas_factor(S, inv, m, i);
+        f[base + i * stride] = y;
+    }
+    double z = 0.0;
+    uint32_t e = m;  // nodes [0, e) are left
+    for (; e >= U; e -= U) {
+        double t[U];
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) t[k] = f[base + (e - 1 - k) * stride];
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) {
+            const uint32_t n = e - 1 - k;
+            const double c = (n == m - 1) ? 0.0 : thomas_factor(S, inv, m, n);
+            z = t[k] - c * z;
+            epilogue(S, base + n * stride, ubase + n * ustride, z);
+        }
+    }
+    for (; e > 0; e--) {
+        const uint32_t n = e - 1;
+        const double c = (n == m - 1) ? 0.0 : thomas_factor(S, inv, m, n);
+        z = f[base + n * stride] - c * z;
+        epilogue(S, base + n * stride, ubase + n * ustride, z);
+    }
+}
+
 // ---- recompose interpolation: u = I u_c + W ------------------------------------
 // One workgroup per fine tile 8 x 8 x 64; its coarse neighbourhood
 // (5 x 5 x 33) is staged in LDS, W is read from the coefficients.
@@ -789,8 +856,8 @@ __global__ __launch_bounds__(256) void k_interp(LoadParams<T> P, const T *__rest
     for (int k = 0; k < kStage; k++)
         if (threadIdx.x + 256u * k < ne) C[sl[k]] = sv[k];
     __syncthreads();
-    // a wave handles rows of 64 along dim 2; every address is a tile base
-    // pointer plus a 32-bit byte offset (row part + lane part)
+    // a wave handles rows of 64 along dim 2; row and plane offsets are
+    // formed in 64 bits, lane offsets inside a row or plane in 32
     const uint32_t lane = threadIdx.x % kI2, wave = threadIdx.x / kI2;
     const uint32_t n0 = P.n[0], n1 = P.n[1], n2 = P.n[2];
     const uint32_t i2 = s2 + lane;
@@ -823,12 +890,15 @@ __global__ __launch_bounds__(256) void k_interp(LoadParams<T> P, const T *__rest
             const bool ok = (FULL || (i0 < n0 && i1 < n1 && in2)) && !coarse;
             T v;
             if (ILV) {
-                // DirectInterleaver order inside the level buffer
+                // DirectInterleaver order inside the level buffer: the plane
+                // base in 64 bits, the offset inside the plane in 32
                 const uint32_t k0 = (i0 >> 1) - h0, p1 = rpos(i1, c[1]);
-                const uint32_t off = !odd0 ? k0 * plane_in + (p1 < c[1] ? p1 * (n2 - c[2]) + (p2 - c[2])
-                                                                        : c[1] * (n2 - c[2]) + (p1 - c[1]) * n2 + p2)
-                                           : k0 * n12 + p1 * n2 + p2;
-                const char *src = (ok && odd0) ? co : ce;
+                const uint32_t off = !odd0 ? (p1 < c[1] ? p1 * (n2 - c[2]) + (p2 - c[2])
+                                                        : c[1] * (n2 - c[2]) + (p1 - c[1]) * n2 + p2)
+                                           : p1 * n2 + p2;
+                const char *src = !ok ? ce
+                                      : odd0 ? co + (uint64_t)k0 * n12 * sizeof(T)
+                                             : ce + (uint64_t)k0 * plane_in * sizeof(T);
                 v = *reinterpret_cast<const T *>(src + (ok ? off * (uint32_t)sizeof(T) : 0u));
             } else {
                 v = P.coef_full[ok ? coef_index_full(P, i0, i1, i2) : 0];
@@ -852,7 +922,7 @@ __global__ __launch_bounds__(256) void k_interp(LoadParams<T> P, const T *__rest
                          ((Cba[la2] + Cba[lb2]) + (Cbb[la2] + Cbb[lb2]))) *
                         (T)0.125;
             const T v = (o0 + o1 + o2 == 0) ? Caa[la2] : (T)(wv[q] + I);
-            *reinterpret_cast<T *>(ot + (r0 * (uint32_t)os0 + r1 * (uint32_t)os1 + lane) * (uint32_t)sizeof(T)) = v;
+            *reinterpret_cast<T *>(ot + ((uint64_t)r0 * os0 + (uint64_t)r1 * os1) * sizeof(T) + lane * (uint32_t)sizeof(T)) = v;
         }
     };
     if (full) body(std::true_type{});
@@ -1054,7 +1124,9 @@ int solve_all(const LevelGeom &g, int l, double *f, const double *tab_dev, const
             case 8: hipLaunchKernelGGL((k_solve_contig<T, 8>), dim3(grid), dim3(256), 0, s, S, nlines); break;
             case 9: hipLaunchKernelGGL((k_solve_contig<T, 9>), dim3(grid), dim3(256), 0, s, S, nlines); break;
             case 16: hipLaunchKernelGGL((k_solve_contig<T, 16>), dim3(grid), dim3(256), 0, s, S, nlines); break;
-            default: set_error("line too long for the contiguous solve"); return kErrUnsupported;
+            default:
+                hipLaunchKernelGGL((k_solve_seq<T>), dim3(div_up(nlines, 256)), dim3(256), 0, s, S, m, nlines, d);
+                break;
             }
         } else {
             const uint64_t nlines = (uint64_t)(d == 1 ? c[0] : c[1]) * S.fp;
@@ -1073,7 +1145,8 @@ int solve_all(const LevelGeom &g, int l, double *f, const double *tab_dev, const
                 hipLaunchKernelGGL((k_solve_strided<T, 40>), dim3(grid), dim3(64 * nw), 0, s, S, m, nlines, d);
             else if (seg <= 72)
                 hipLaunchKernelGGL((k_solve_strided<T, 72>), dim3(grid), dim3(64 * nw), 0, s, S, m, nlines, d);
-            else { set_error("line too long for the strided solve"); return kErrUnsupported; }
+            else
+                hipLaunchKernelGGL((k_solve_seq<T>), dim3(div_up(nlines, 256)), dim3(256), 0, s, S, m, nlines, d);
         }
         PRODM_CHECK_LAUNCH();
     }
@@ -1093,7 +1166,9 @@ LoadParams<T> load_params(const LevelGeom &g, int l, const CoefSink<T> &sink) {
 // The decompose and adjoint sweeps address an input plane, a coefficient
 // plane and a coarse-value plane through buffer descriptors of range
 // 2^31 - 1 bytes (masked lanes use the offset 2^31), so each plane's byte
-// extent must stay below 2^31.
+// extent must stay below 2^31. The recompose sweep and the interpolation
+// form offsets inside a plane in 32 bits; they keep the same limit, so that
+// one rule covers a plan's refactor and retrieve.
 template <class T> bool sweep_planes_fit(const LoadParams<T> &P) {
     const uint64_t lim = 0x7FFFFFFFull;
     const uint64_t in_plane = ((uint64_t)(P.n[1] - 1) * P.is[1] + P.n[2]) * sizeof(T);
@@ -1103,7 +1178,7 @@ template <class T> bool sweep_planes_fit(const LoadParams<T> &P) {
 
 template <class T, int MODE>
 int launch_load(LoadParams<T> P, bool ilv, hipStream_t s) {
-    if (MODE != kRecompose && !sweep_planes_fit(P)) {
+    if (!sweep_planes_fit(P)) {
         set_error("a fine plane of this level spans 2 GiB or more; the sweeps address planes with 31-bit offsets");
         return kErrUnsupported;
     }

@@ -18,7 +18,12 @@ GEOMS = [([129], 5), ([33, 65], 4), ([65, 33], 3), ([17, 33, 9], 2), ([65, 65, 6
          # contiguous lines of 257, 401, 513 and 801 coarse nodes: 5, 7, 9 and
          # 13 nodes per lane in the dim-2 solve; in 1-D that solve also
          # updates the coarse values
-         ([5, 513], 1), ([3, 801], 1), ([3, 1025], 1), ([3, 1601], 1), ([1025], 2), ([1601], 1)]
+         ([5, 513], 1), ([3, 801], 1), ([3, 1025], 1), ([3, 1601], 1), ([1025], 2), ([1601], 1),
+         # lines past the register-resident solves (dim 2: 1024 coarse nodes,
+         # dims 1 / 0: 576) take the sequential lane-per-line solve: 1025
+         # coarse nodes along each dim, 577 against 576 along dim 0
+         ([2049], 1), ([2049], 3), ([3, 2049], 1), ([2049, 5], 1), ([2049, 2049], 2), ([2049, 5, 9], 2),
+         ([1153, 3, 5], 1), ([1151, 3, 5], 1), ([9, 2049, 17], 1)]
 
 
 def rel(a, b):
@@ -120,3 +125,42 @@ def test_plan_hierarchical_round_trip(gpu):
     out = plan.retrieve([nb] * (L + 1)).cpu().numpy()
     assert np.max(np.abs(out.astype(np.float64) - u)) <= 1e-5 * np.max(np.abs(u))
     plan.close()
+
+
+@pytest.mark.parametrize("decomposer", [0, 1])
+def test_wide_planes_round_trip(gpu, decomposer):
+    """9 planes of 12545 x 12545 fp32 nodes (630 MB each; an interpolation
+    tile's 8 planes span more than 4 GiB, so its row and plane offsets are
+    formed in 64 bits). A full-precision retrieval through a plan
+    (interleaved level buffers) and the standalone decompose / recompose both
+    return the field. The orthogonal decomposer's dim-2 and dim-1 lines hold
+    6273 coarse nodes (the sequential lane-per-line solve)."""
+    from prodm_amd import capi, mdr
+    dims, L, nb = [9, 12545, 12545], 1, 32
+    assert 7 * dims[1] * dims[2] * 4 >= 2 ** 32
+    n = int(np.prod(dims))
+    a = torch.linspace(0.0, 7.0, dims[1], device=gpu, dtype=torch.float64)
+    b = torch.linspace(0.0, 5.0, dims[2], device=gpu, dtype=torch.float64)
+    t = torch.linspace(-0.5, 0.5, dims[0], device=gpu, dtype=torch.float64)
+    u = (torch.sin(a)[None, :, None] * torch.cos(1.3 * b)[None, None, :] + t[:, None, None]).float().reshape(-1)
+    del a, b
+    top = torch.max(torch.abs(u)).item()
+    plan = mdr.Plan(dims, L, nb, torch.float32, capi.NEGABINARY, decomposer=decomposer)
+    plan.refactor(u)
+    out = plan.retrieve([nb] * (L + 1))
+    err = torch.max(torch.abs(out - u)).item()
+    print(f"wide planes decomposer={decomposer} plan max err {err:.3e} of {top:.3e}", flush=True)
+    assert err <= 1e-5 * top
+    # every row and plane of the output was written (a wrapped offset would
+    # leave rows of the preset value)
+    out.fill_(float("nan"))
+    plan.retrieve([nb] * (L + 1), out=out)
+    assert not torch.isnan(out).any().item()
+    plan.close()
+    del out
+    hier = decomposer == 1
+    c = mdr.decompose(u, dims, L, hierarchical=hier)
+    r = mdr.recompose(c, dims, L, hierarchical=hier)
+    assert r.numel() == n
+    err = torch.max(torch.abs(r - u)).item()
+    assert err <= 1e-5 * top, err

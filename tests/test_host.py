@@ -47,6 +47,33 @@ def test_version_and_error_channel():
     assert b"bad arguments" in lib.prodm_last_error()
 
 
+def test_planes_of_2gib_refused_before_any_device_call():
+    """The level sweeps address a plane of the last two dims with 31-bit byte
+    offsets, so a plan (and a standalone decompose or recompose) refuses a
+    geometry whose plane spans 2 GiB or more, with a message, before any
+    device call: this runs without a GPU. Planes just under the limit are
+    accepted by the geometry check."""
+    import ctypes as C
+    lib = capi.lib()
+    plan = C.c_void_p()
+    for dims, dt in (([3, 32769, 16385], 0), ([3, 16385, 16385], 1), ([32769, 16385], 0)):
+        d = np.array(dims, np.uint32)
+        for enc in (0, 1):
+            rc = lib.prodm_mdr_plan_create(C.byref(plan), d.ctypes.data_as(C.c_void_p), len(dims), 1, 32, dt, enc)
+            assert rc == -6 and not plan.value, (dims, dt, rc)
+            assert b"2 GiB" in lib.prodm_last_error()
+        dec = lib.prodm_decompose_f64 if dt else lib.prodm_decompose_f32
+        rec = lib.prodm_recompose_f64 if dt else lib.prodm_recompose_f32
+        for fn in (dec, rec):
+            rc = fn(None, None, d.ctypes.data_as(C.c_void_p), len(dims), 1, None, None, 0, None)
+            assert rc == -6 and b"2 GiB" in lib.prodm_last_error()
+    # one element short of the limit passes the geometry check and fails on
+    # the null pointers instead
+    d = np.array([3, 16383, 32767], np.uint32)  # 16383 * 32767 * 4 < 2^31 - 1
+    rc = lib.prodm_decompose_f32(None, None, d.ctypes.data_as(C.c_void_p), 3, 1, None, None, 0, None)
+    assert rc == -1 and b"distinct device input" in lib.prodm_last_error()
+
+
 @pytest.mark.parametrize("dims,L", [([513, 513, 513], 6), ([1025, 1025, 257], 7), ([129, 129, 129], 4),
                                     ([65], 5), ([33, 17], 3), ([1, 65, 33], 4)])
 def test_level_geometry_matches_oracle(dims, L):
