@@ -166,7 +166,7 @@ struct GroupShape {
 struct GroupLaunch {
     int kind = 0;
     int elem = 4;       // bytes of the field type
-    int variant = 0;    // kernel instance (register segment length of a solve)
+    int variant = 0;    // kernel instance (contiguous solve: largest class; strided: register segment length)
     int nb = 0;         // encode: bitplanes
     uint32_t block = 256, nent = 0, total = 0;
     GroupShape shape;

@@ -527,29 +527,38 @@ template <class T> __device__ inline void epilogue(const SolveParams<T> &S, uint
     else ((PRODM_GLOBAL double *)S.f)[fidx] = z;
 }
 
-// contiguous lines (dim 2): a wave solves kContigLPW lines side by side (two
-// independent chains interleave, hiding each other's latency), each lane
-// holding a segment of at most SEG of every line; wave w of the workgroup
-// solves lines line0 + kContigLPW w + p
+// contiguous lines (dim 2): a wave's lanes form groups of W (64, or 32 / 16
+// for lines of at most 32 / 16 nodes, so that short lines do not leave most
+// lanes idle); a group solves kContigLPW lines side by side (two independent
+// chains interleave, hiding each other's latency), each lane holding a
+// segment of at most SEG nodes of every line. Wave w of the workgroup solves
+// the kContigLPW * 64 / W lines from line0 + w * kContigLPW * 64 / W, lane
+// group g of it lines + g * kContigLPW + p. The scans stay inside a group: for
+// a line that fits a narrower group, the steps a 64-wide scan adds combine
+// only with identity elements of idle lanes, so the result is the same.
 constexpr int kContigLPW = 2;
-template <class T, int SEG>
+template <int W> constexpr uint32_t contig_lines_per_wave() { return kContigLPW * (64 / W); }
+template <class T, int SEG, int W>
 __device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint64_t nlines, uint64_t line0) {
     constexpr int NL = kContigLPW;
+    static_assert(W == 16 || W == 32 || W == 64, "lane group width");
     const uint32_t m = S.c[2];
     const uint32_t lane = lane_id(), wv = threadIdx.x / 64;
-    const uint64_t first = line0 + (uint64_t)wv * NL;
-    if (first >= nlines) return;
-    const uint32_t seg = (m + 63) / 64;
-    const uint32_t b = lane * seg;
+    const uint32_t li = lane % W, grp = lane / W;
+    const uint64_t wfirst = line0 + (uint64_t)wv * contig_lines_per_wave<W>();
+    if (wfirst >= nlines) return;
+    const uint64_t first = wfirst + (uint64_t)grp * NL;  // this lane group's first line
+    const uint32_t seg = (m + W - 1) / W;
+    const uint32_t b = li * seg;
     const uint32_t e = min(m, b + seg);
     // Slot i of the lane is node b + i when in[i]. The factors are loaded
     // once; slots past the segment compute on zeros and are discarded by
     // selects (no exec-mask branches), and their loads and stores carry
-    // kOffNone. Lines past nlines get zero-range descriptors.
+    // kOffNone. Lines past nlines get zero-range descriptors (W = 64) or
+    // kOffNone offsets (narrower groups share one descriptor per wave).
     bool in[SEG];
     double iv[SEG];
     const auto rinv = buf_rsrc((const void *)S.inv, true);
-    auto off = [&](int i) __attribute__((always_inline)) -> uint32_t { return in[i] ? (b + i) * 8u : kOffNone; };
 #pragma unroll
     for (int i = 0; i < SEG; i++) {
         const uint32_t k = b + i;
@@ -559,13 +568,26 @@ __device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint
     }
     auto fac = [&](int i) __attribute__((always_inline)) -> double { return iv[i]; };
     __amdgpu_buffer_rsrc_t rl[NL];
-    double v[NL][SEG];
+    uint32_t lo[NL];  // byte offset of line p inside its descriptor (kOffNone: past nlines)
 #pragma unroll
     for (int p = 0; p < NL; p++) {
         const bool ok = first + p < nlines;
-        rl[p] = buf_rsrc((const void *)((const PRODM_GLOBAL double *)S.f + (ok ? first + p : 0) * S.fp), ok);
+        if constexpr (W == 64) {
+            rl[p] = buf_rsrc((const void *)((const PRODM_GLOBAL double *)S.f + (ok ? first + p : 0) * S.fp), ok);
+            lo[p] = 0;
+        } else {
+            rl[p] = buf_rsrc((const void *)((const PRODM_GLOBAL double *)S.f + wfirst * S.fp), true);
+            lo[p] = ok ? (uint32_t)((first + p - wfirst) * S.fp) * 8u : kOffNone;
+        }
+    }
+    auto off = [&](int p, int i) __attribute__((always_inline)) -> uint32_t {
+        return (in[i] && lo[p] != kOffNone) ? lo[p] + (b + i) * 8u : kOffNone;
+    };
+    double v[NL][SEG];
 #pragma unroll
-        for (int i = 0; i < SEG; i++) v[p][i] = buf_load<double>(rl[p], off(i));
+    for (int p = 0; p < NL; p++) {
+#pragma unroll
+        for (int i = 0; i < SEG; i++) v[p][i] = buf_load<double>(rl[p], off(p, i));
     }
     // forward: y_i = (f_i - y_{i-1}) * inv_i  == affine map y_{i-1} -> y_i
     double A[NL], Bm[NL];
@@ -581,18 +603,18 @@ __device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint
             Bm[p] = in[i] ? nB : Bm[p];
         }
     }
-    for (int o = 1; o < 64; o <<= 1) {
+    for (int o = 1; o < W; o <<= 1) {
 #pragma unroll
         for (int p = 0; p < NL; p++) {
-            const double pA = __shfl_up(A[p], o, 64), pB = __shfl_up(Bm[p], o, 64);
-            if ((int)lane >= o) { A[p] = A[p] + Bm[p] * pA; Bm[p] = Bm[p] * pB; }
+            const double pA = __shfl_up(A[p], o, W), pB = __shfl_up(Bm[p], o, W);
+            if ((int)li >= o) { A[p] = A[p] + Bm[p] * pA; Bm[p] = Bm[p] * pB; }
         }
     }
     double y[NL];
 #pragma unroll
     for (int p = 0; p < NL; p++) {
-        y[p] = __shfl_up(A[p], 1, 64);
-        if (lane == 0) y[p] = 0.0;
+        y[p] = __shfl_up(A[p], 1, W);
+        if (li == 0) y[p] = 0.0;
     }
 #pragma unroll
     for (int i = 0; i < SEG; i++) {
@@ -618,18 +640,18 @@ __device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint
             Dm[p] = in[i] ? nD : Dm[p];
         }
     }
-    for (int o = 1; o < 64; o <<= 1) {
+    for (int o = 1; o < W; o <<= 1) {
 #pragma unroll
         for (int p = 0; p < NL; p++) {
-            const double nC = __shfl_down(Cc[p], o, 64), nD = __shfl_down(Dm[p], o, 64);
-            if (lane + o < 64) { Cc[p] = Cc[p] + Dm[p] * nC; Dm[p] = Dm[p] * nD; }
+            const double nC = __shfl_down(Cc[p], o, W), nD = __shfl_down(Dm[p], o, W);
+            if (li + o < W) { Cc[p] = Cc[p] + Dm[p] * nC; Dm[p] = Dm[p] * nD; }
         }
     }
     double z[NL];
 #pragma unroll
     for (int p = 0; p < NL; p++) {
-        z[p] = __shfl_down(Cc[p], 1, 64);
-        if (lane == 63) z[p] = 0.0;
+        z[p] = __shfl_down(Cc[p], 1, W);
+        if (li == W - 1) z[p] = 0.0;
     }
 #pragma unroll
     for (int i = SEG - 1; i >= 0; i--) {
@@ -645,7 +667,7 @@ __device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint
 #pragma unroll
         for (int p = 0; p < NL; p++)
 #pragma unroll
-            for (int i = 0; i < SEG; i++) buf_store<double>(rl[p], off(i), v[p][i]);
+            for (int i = 0; i < SEG; i++) buf_store<double>(rl[p], off(p, i), v[p][i]);
         return;
     }
 #pragma unroll
@@ -657,29 +679,44 @@ __device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint
             if (in[i]) epilogue(S, line * S.fp + b + i, line * m + b + i, v[p][i]);
     }
 }
-template <class T, int SEG>
+template <class T, int SEG, int W>
 __global__ __launch_bounds__(256) void k_solve_contig(SolveParams<T> S, uint64_t nlines) {
-    solve_contig_lines<T, SEG>(S, nlines, (uint64_t)blockIdx.x * 4 * kContigLPW);
+    solve_contig_lines<T, SEG, W>(S, nlines, (uint64_t)blockIdx.x * 4 * contig_lines_per_wave<W>());
 }
+
+// contiguous-solve classes by line length: class c holds lines of up to
+// kContigSeg[c] * kContigW[c] nodes (0: longer than the register-resident
+// solve holds)
+constexpr int kContigSeg[10] = {0, 1, 1, 1, 2, 4, 5, 8, 9, 16};
+constexpr int kContigW[10] = {0, 16, 32, 64, 64, 64, 64, 64, 64, 64};
+inline int contig_class(uint32_t m) {
+    for (int c = 1; c < 10; c++)
+        if (m <= (uint32_t)(kContigSeg[c] * kContigW[c])) return c;
+    return 0;
+}
+inline uint32_t contig_class_lines_per_wave(int c) { return kContigLPW * (64 / kContigW[c]); }
 
 // strided lines along dim d (1 or 0) of the coarse grid, lane per line.
 // Lines are indexed over padded rows, q = r * fp + j2 (r = j0 for d = 1,
 // j1 for d = 0); lanes with j2 >= c2 idle, so each wave's 64 accesses are
 // one aligned 512-byte run of f. nw waves split the line into segments of
-// at most SEG (nw is a function of m alone, so the scan order is fixed);
-// waves of the workgroup past nw find empty segments and only join the
-// barriers. bid: the workgroup's 64-line group.
+// at most SEG (nw is a function of m alone, so the scan order is fixed).
+// A workgroup may hold several such groups (the grouped launches pack them):
+// this wave is wave wv of the group whose waves are wbase .. wbase + nw - 1
+// of the workgroup; a wave with gact false (past the packed groups) finds an
+// empty segment and only joins the barriers. bid: the group's 64-line group.
 template <class T, int SEG>
 __device__ __forceinline__ void solve_strided_lines(const SolveParams<T> &S, uint32_t m, uint64_t nlines, int d,
-                                                    uint32_t nw, uint32_t bid) {
+                                                    uint32_t nw, uint32_t bid, uint32_t wv, uint32_t wbase,
+                                                    bool gact) {
     __shared__ double sA[8][64], sB[8][64];
     const uint32_t lane = lane_id();
-    const uint32_t wv = threadIdx.x / 64;
+    const uint32_t w0 = wbase + wv;  // this wave's row of the scan buffers
     const uint64_t q = (uint64_t)bid * 64 + lane;
     const uint32_t fp = S.fp, c1 = S.c[1], c2 = S.c[2];
     const uint32_t j2 = (uint32_t)(q % fp);
     const uint64_t r = q / fp;
-    const bool act = q < nlines && j2 < c2;
+    const bool act = gact && q < nlines && j2 < c2;
     uint64_t base = 0, stride, ubase = 0, ustride;
     if (d == 1) {
         stride = fp;
@@ -697,7 +734,7 @@ __device__ __forceinline__ void solve_strided_lines(const SolveParams<T> &S, uin
     const uint32_t seg = (m + nw - 1) / nw;
     // wave-uniform segment bounds: the factor loads become scalar loads
     const uint32_t b = (uint32_t)__builtin_amdgcn_readfirstlane((int)(wv * seg));
-    const uint32_t e = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(m, b + seg));
+    const uint32_t e = gact ? (uint32_t)__builtin_amdgcn_readfirstlane((int)min(m, b + seg)) : b;
     double v[SEG];
 #pragma unroll
     for (int i = 0; i < SEG; i++) v[i] = (act && b + i < e) ? fg[base + (uint64_t)(b + i) * stride] : 0.0;
@@ -710,11 +747,11 @@ __device__ __forceinline__ void solve_strided_lines(const SolveParams<T> &S, uin
             Bm = -Bm * iv;
         }
     }
-    sA[wv][lane] = A;
-    sB[wv][lane] = Bm;
+    sA[w0][lane] = A;
+    sB[w0][lane] = Bm;
     __syncthreads();
     double y = 0.0;
-    for (uint32_t w = 0; w < wv; w++) y = sA[w][lane] + sB[w][lane] * y;
+    for (uint32_t w = 0; w < wv; w++) y = sA[wbase + w][lane] + sB[wbase + w][lane] * y;
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < SEG; i++) {
@@ -732,11 +769,12 @@ __device__ __forceinline__ void solve_strided_lines(const SolveParams<T> &S, uin
             Dm = -c * Dm;
         }
     }
-    sA[wv][lane] = C;
-    sB[wv][lane] = Dm;
+    sA[w0][lane] = C;
+    sB[w0][lane] = Dm;
     __syncthreads();
     double z = 0.0;
-    for (int w = (int)nw - 1; w > (int)wv; w--) z = sA[w][lane] + sB[w][lane] * z;
+    if (gact)
+        for (int w = (int)nw - 1; w > (int)wv; w--) z = sA[wbase + w][lane] + sB[wbase + w][lane] * z;
     // each node's result is stored as soon as it is known (its epilogue
     // loads do not wait for the end of the segment)
 #pragma unroll
@@ -750,7 +788,7 @@ __device__ __forceinline__ void solve_strided_lines(const SolveParams<T> &S, uin
 }
 template <class T, int SEG>
 __global__ __launch_bounds__(512) void k_solve_strided(SolveParams<T> S, uint32_t m, uint64_t nlines, int d) {
-    solve_strided_lines<T, SEG>(S, m, nlines, d, blockDim.x / 64, blockIdx.x);
+    solve_strided_lines<T, SEG>(S, m, nlines, d, blockDim.x / 64, blockIdx.x, threadIdx.x / 64, 0, true);
 }
 
 // Lines longer than the register-resident solves hold (dim 2: more than
@@ -956,7 +994,7 @@ template <class T> struct SolveEntry {
     SolveParams<T> S;
     uint64_t nlines;
     uint32_t m, nw;   // line length, waves per line (strided)
-    int32_t d, pad;
+    int32_t d, cls;   // solve dim, contiguous-solve class (dim 2)
 };
 template <class T> struct Level0Entry {
     const T *src;     // coarsest grid (strides s0, s1)
@@ -979,7 +1017,23 @@ __global__ __launch_bounds__(kLoadThreads) void k_load_grouped(const LoadParams<
     }
 }
 
-template <class T, int SEG>
+// each entry runs its own contiguous-solve class (lane group width and
+// slots per lane), up to the launch's largest class MAXC
+template <class T, int C>
+__device__ __forceinline__ void contig_class_lines(const SolveEntry<T> &E, uint32_t wpb, uint32_t vbl) {
+    solve_contig_lines<T, kContigSeg[C], kContigW[C]>(E.S, E.nlines,
+                                                        (uint64_t)vbl * wpb * contig_lines_per_wave<kContigW[C]>());
+}
+template <class T, int C, int MAXC>
+__device__ __forceinline__ void contig_dispatch(int c, const SolveEntry<T> &E, uint32_t wpb, uint32_t vbl) {
+    if constexpr (C < MAXC) {
+        if (c == C) contig_class_lines<T, C>(E, wpb, vbl);
+        else contig_dispatch<T, C + 1, MAXC>(c, E, wpb, vbl);
+    } else {
+        contig_class_lines<T, MAXC>(E, wpb, vbl);  // c == MAXC: no entry exceeds the launch's class
+    }
+}
+template <class T, int MAXC>
 __global__ __launch_bounds__(512) void k_contig_grouped(const SolveEntry<T> *__restrict__ ps,
                                                         const uint32_t *__restrict__ first, uint32_t nent,
                                                         uint32_t total) {
@@ -987,20 +1041,25 @@ __global__ __launch_bounds__(512) void k_contig_grouped(const SolveEntry<T> *__r
     for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
         const uint32_t e = group_entry(first, nent, vb);
         const SolveEntry<T> E = uniform_load(ps + e);
-        const uint32_t f0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)first[e]);
-        solve_contig_lines<T, SEG>(E.S, E.nlines, (uint64_t)(vb - f0) * wpb * kContigLPW);
+        const uint32_t vbl = vb - (uint32_t)__builtin_amdgcn_readfirstlane((int)first[e]);
+        contig_dispatch<T, 1, MAXC>(__builtin_amdgcn_readfirstlane(E.cls), E, wpb, vbl);
     }
 }
 
+// A workgroup of 64 * WB threads holds floor(WB / nw) of an entry's line
+// groups (nw waves each); the scan order of each is the single-field one.
 template <class T, int SEG>
 __global__ __launch_bounds__(512) void k_strided_grouped(const SolveEntry<T> *__restrict__ ps,
                                                          const uint32_t *__restrict__ first, uint32_t nent,
                                                          uint32_t total) {
+    const uint32_t w = threadIdx.x / 64, wb = blockDim.x / 64;
     for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
         const uint32_t e = group_entry(first, nent, vb);
         const SolveEntry<T> E = uniform_load(ps + e);
         const uint32_t f0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)first[e]);
-        solve_strided_lines<T, SEG>(E.S, E.m, E.nlines, E.d, E.nw, vb - f0);
+        const uint32_t nw = (uint32_t)__builtin_amdgcn_readfirstlane((int)E.nw);
+        const uint32_t G = wb / nw, g = w / nw;
+        solve_strided_lines<T, SEG>(E.S, E.m, E.nlines, E.d, nw, (vb - f0) * G + g, w - g * nw, g * nw, g < G);
         __syncthreads();  // the cross-wave scan buffers are reused
     }
 }
@@ -1092,7 +1151,6 @@ void set_factors(SolveParams<T> &S, const double *tab_dev, const Factors &F, int
     }
 }
 
-int seg_class_contig(uint32_t m);  // registers per lane of the contiguous solve (0: line too long)
 
 template <class T>
 int solve_all(const LevelGeom &g, int l, double *f, const double *tab_dev, const Factors &F, int epi, const T *uc_in,
@@ -1116,18 +1174,24 @@ int solve_all(const LevelGeom &g, int l, double *f, const double *tab_dev, const
         const uint32_t m = c[d];
         if (d == 2) {
             const uint64_t nlines = (uint64_t)c[0] * c[1];
-            const uint32_t grid = div_up(nlines, 4 * kContigLPW);
-            const uint32_t seg = (m + 63) / 64;
-            switch (seg_class_contig(m)) {
-            case 4: hipLaunchKernelGGL((k_solve_contig<T, 4>), dim3(grid), dim3(256), 0, s, S, nlines); break;
-            case 5: hipLaunchKernelGGL((k_solve_contig<T, 5>), dim3(grid), dim3(256), 0, s, S, nlines); break;
-            case 8: hipLaunchKernelGGL((k_solve_contig<T, 8>), dim3(grid), dim3(256), 0, s, S, nlines); break;
-            case 9: hipLaunchKernelGGL((k_solve_contig<T, 9>), dim3(grid), dim3(256), 0, s, S, nlines); break;
-            case 16: hipLaunchKernelGGL((k_solve_contig<T, 16>), dim3(grid), dim3(256), 0, s, S, nlines); break;
+            const int cls = contig_class(m);
+            const uint32_t grid = cls ? div_up(nlines, 4 * contig_class_lines_per_wave(cls)) : 0;
+#define PRODM_CONTIG(SEG, W) hipLaunchKernelGGL((k_solve_contig<T, SEG, W>), dim3(grid), dim3(256), 0, s, S, nlines)
+            switch (cls) {
+            case 1: PRODM_CONTIG(1, 16); break;
+            case 2: PRODM_CONTIG(1, 32); break;
+            case 3: PRODM_CONTIG(1, 64); break;
+            case 4: PRODM_CONTIG(2, 64); break;
+            case 5: PRODM_CONTIG(4, 64); break;
+            case 6: PRODM_CONTIG(5, 64); break;
+            case 7: PRODM_CONTIG(8, 64); break;
+            case 8: PRODM_CONTIG(9, 64); break;
+            case 9: PRODM_CONTIG(16, 64); break;
             default:
                 hipLaunchKernelGGL((k_solve_seq<T>), dim3(div_up(nlines, 256)), dim3(256), 0, s, S, m, nlines, d);
                 break;
             }
+#undef PRODM_CONTIG
         } else {
             const uint64_t nlines = (uint64_t)(d == 1 ? c[0] : c[1]) * S.fp;
             // up to 8 waves split a line; each lane keeps its segment in registers
@@ -1427,10 +1491,6 @@ template <class E> void fill_sorted(GroupLaunch &L, std::vector<std::pair<uint32
         if (x.first) append_entry(L, x.second, x.first);
 }
 
-int seg_class_contig(uint32_t m) {
-    const uint32_t seg = (m + 63) / 64;
-    return seg <= 4 ? 4 : seg == 5 ? 5 : seg <= 8 ? 8 : seg == 9 ? 9 : seg <= 16 ? 16 : 0;
-}
 int seg_class_strided(uint32_t seg) {
     return seg <= 8 ? 8 : seg <= 16 ? 16 : seg <= 24 ? 24 : seg <= 40 ? 40 : seg <= 72 ? 72 : 0;
 }
@@ -1534,6 +1594,7 @@ int mgard_batched_launches(const std::vector<BatchFieldRef> &fields, const std::
                     E.d = d;
                     E.nlines = d == 2 ? (uint64_t)c[0] * c[1] : (uint64_t)(d == 1 ? c[0] : c[1]) * E.S.fp;
                     E.nw = d == 2 ? 1 : strided_waves(E.m);
+                    E.cls = d == 2 ? contig_class(E.m) : 0;
                     se[d].push_back({0, E});
                 }
             }
@@ -1551,13 +1612,18 @@ int mgard_batched_launches(const std::vector<BatchFieldRef> &fields, const std::
                 GroupLaunch L = make(kGrpContig, 256);
                 const uint32_t wpb = L.shape.a > 0 ? std::min(8, L.shape.a) : 4;
                 L.block = 64 * wpb;
-                uint32_t mmax = 0;
+                int cmax = 1;
                 for (auto &x : se[d]) {
-                    mmax = std::max(mmax, x.second.m);
-                    x.first = div_up(x.second.nlines, (uint64_t)wpb * kContigLPW);
+                    const int c = x.second.cls;
+                    if (!c) {
+                        set_error("batched decompose: a contiguous line of more than 1024 coarse nodes (the grouped "
+                                  "solves hold lines in registers)");
+                        return kErrUnsupported;
+                    }
+                    cmax = std::max(cmax, c);
+                    x.first = div_up(x.second.nlines, (uint64_t)wpb * contig_class_lines_per_wave(c));
                 }
-                L.variant = seg_class_contig(mmax);
-                if (!L.variant) { set_error("line too long for the contiguous solve"); return kErrUnsupported; }
+                L.variant = cmax;
                 fill_sorted(L, se[d]);
                 out.push_back(std::move(L));
             } else {
@@ -1567,10 +1633,16 @@ int mgard_batched_launches(const std::vector<BatchFieldRef> &fields, const std::
                     const uint32_t nw = x.second.nw;
                     segmax = std::max(segmax, (x.second.m + nw - 1) / nw);
                     nwmax = std::max(nwmax, nw);
-                    x.first = div_up(x.second.nlines, 64);
                 }
+                // a workgroup of nwmax waves packs floor(nwmax / nw) of an
+                // entry's 64-line groups
+                for (auto &x : se[d]) x.first = div_up(div_up(x.second.nlines, 64), nwmax / x.second.nw);
                 L.variant = seg_class_strided(segmax);
-                if (!L.variant) { set_error("line too long for the strided solve"); return kErrUnsupported; }
+                if (!L.variant) {
+                    set_error("batched decompose: a strided line of more than 576 coarse nodes (the grouped solves "
+                              "hold lines in registers)");
+                    return kErrUnsupported;
+                }
                 L.block = 64 * nwmax;
                 fill_sorted(L, se[d]);
                 out.push_back(std::move(L));
@@ -1610,12 +1682,18 @@ template <class T> int group_run_t(const GroupLaunch &L, hipStream_t s) {
     switch (L.kind) {
     case kGrpLoad: PRODM_GRP((k_load_grouped<T>), LoadParams<T>); break;
     case kGrpContig:
-        if (L.variant == 4) PRODM_GRP((k_contig_grouped<T, 4>), SolveEntry<T>);
-        else if (L.variant == 5) PRODM_GRP((k_contig_grouped<T, 5>), SolveEntry<T>);
-        else if (L.variant == 8) PRODM_GRP((k_contig_grouped<T, 8>), SolveEntry<T>);
-        else if (L.variant == 9) PRODM_GRP((k_contig_grouped<T, 9>), SolveEntry<T>);
-        else if (L.variant == 16) PRODM_GRP((k_contig_grouped<T, 16>), SolveEntry<T>);
-        else { set_error("grouped contiguous solve: bad variant"); return kErrArg; }
+        switch (L.variant) {
+        case 1: PRODM_GRP((k_contig_grouped<T, 1>), SolveEntry<T>); break;
+        case 2: PRODM_GRP((k_contig_grouped<T, 2>), SolveEntry<T>); break;
+        case 3: PRODM_GRP((k_contig_grouped<T, 3>), SolveEntry<T>); break;
+        case 4: PRODM_GRP((k_contig_grouped<T, 4>), SolveEntry<T>); break;
+        case 5: PRODM_GRP((k_contig_grouped<T, 5>), SolveEntry<T>); break;
+        case 6: PRODM_GRP((k_contig_grouped<T, 6>), SolveEntry<T>); break;
+        case 7: PRODM_GRP((k_contig_grouped<T, 7>), SolveEntry<T>); break;
+        case 8: PRODM_GRP((k_contig_grouped<T, 8>), SolveEntry<T>); break;
+        case 9: PRODM_GRP((k_contig_grouped<T, 9>), SolveEntry<T>); break;
+        default: set_error("grouped contiguous solve: bad class"); return kErrArg;
+        }
         break;
     case kGrpStrided:
         if (L.variant == 8) PRODM_GRP((k_strided_grouped<T, 8>), SolveEntry<T>);

@@ -88,6 +88,49 @@ def test_batched_equals_plain(gpu, dt, enc, nb, decomp):
         assert_same_results(b.plan(f), plans[f], L, nb, elem, enc, f"field {f} {d}")
 
 
+@pytest.mark.parametrize("dt,enc,nb", [(np.float32, 0, 32), (np.float64, 1, 60)])
+def test_batched_fields_match_oracle(gpu, dt, enc, nb):
+    """Every batched field against the oracle directly (not through the plain
+    GPU path): level coefficients against the oracle's decompose + interleave
+    (1e-6 relative for fp32, 1e-14 for fp64), level maxima and exponents, and
+    the bitplane streams bit-exact against the oracle encoders fed the
+    batched coefficients. The shapes include lines of 9 and 17 coarse nodes
+    (16- and 32-lane groups of the contiguous solve) and strided lines packed
+    several groups to a workgroup."""
+    import oracle_bind as ob
+    from prodm_amd.batched import Batch
+    tol = 1e-6 if dt == np.float32 else 1e-14
+    us = [bumps_field(d, seed=500 + i, dtype=dt) for i, (d, L) in enumerate(MIXED)]
+    b = Batch([(torch.from_numpy(u).to(gpu), d, L) for u, (d, L) in zip(us, MIXED)], nb, TD[dt], enc)
+    b.refactor()
+    torch.cuda.synchronize()
+    for f, ((d, L), u) in enumerate(zip(MIXED, us)):
+        plan = b.plan(f)
+        dec = ob.decompose(u, d, L)
+        ld = ob.level_dims(d, L)
+        scale = float(np.max(np.abs(dec)))
+        for l in range(L + 1):
+            coarse = ld[l - 1] if l else np.zeros(len(d), np.uint32)
+            want = ob.interleave(dec, d, ld[l], coarse)
+            c = plan.coefficients(l).cpu().numpy()
+            assert c.size == want.size
+            assert np.max(np.abs(c.astype(np.float64) - want)) <= tol * scale, (f, d, l)
+            m, e, n = plan.level_info(l)
+            assert m == ob.maxabs(c) and e == ob.frexp_exp(m, dt), (f, l)
+            if enc == 0:
+                op = ob.nb_encode(c, e, nb)
+                for p in range(nb):
+                    assert plan.plane_bytes(l, p) == op[p].tobytes(), (f, l, p)
+            else:
+                os_ = ob.gb_encode(c, e, nb)
+                rb = plan.level_rb(l)
+                assert os_[0][4:4 + rb.size] == rb.tobytes(), (f, l)
+                for p in range(nb):
+                    want_p = os_[p][4 + rb.size:] if p == 0 else os_[p]
+                    assert plan.plane_bytes(l, p) == want_p, (f, l, p)
+    b.close()
+
+
 def test_batched_graph_autotune_and_cache(gpu, tmp_path):
     from prodm_amd.batched import Batch
     shapes = MIXED[:7]
@@ -145,6 +188,13 @@ def test_batched_rejects_bad_fields(gpu):
         Batch([(x, [33, 33], 9)], 32)          # too many levels for the shape
     with pytest.raises(ValueError):
         Batch([(x, [33, 34], 4)], 32)          # element count mismatch
+    # the grouped solves hold a line in registers: a 2049-point axis (1025
+    # coarse nodes) is refused with a message (the single-field plan takes it)
+    y = torch.zeros(3 * 2049, device=gpu)
+    b = Batch([(y, [3, 2049], 1)], 32)
+    with pytest.raises(capi.ProdmError, match="1024 coarse nodes"):
+        b.refactor()  # the launch tables are built (and checked) before anything runs
+    b.close()
 
 
 def test_batched_full_workload(gpu):
