@@ -166,22 +166,31 @@ template <class T> __host__ __device__ inline uint32_t load_tiles(const LoadPara
     return t0n * t1n * t2n;
 }
 
-// the sweep of coarse tile bt (one workgroup)
-template <class T, int MODE, bool ILV>
-__device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt) {
+// LDS of a sweep: staged fine planes, W planes and dim-2 stencils. The
+// kernels declare it, so that the sweep's forms in one kernel share it.
+template <class T, int MODE> struct SweepLds {
+    static constexpr bool kDec = MODE == kDecompose;
+    static constexpr int kPE = kDec ? kPL : 1;  // input planes staged only by decompose
+    T E[kPE], O[kPE], Wb[kPL], Wb2[kPE];
+    double G2[kG2], G2b[kDec ? kG2 : 1];
+};
+
+// the sweep of coarse tile bt (one workgroup). A3: every dim of the level is
+// longer than 1 (the tests of inactive dims fold away; same arithmetic)
+template <class T, int MODE, bool ILV, bool A3 = false>
+__device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt, SweepLds<T, MODE> &lds) {
     using K = KeyOf<T>;
     // buffer accesses through global-address-space pointers: the parameters
     // may come from memory (grouped launches), where the compiler cannot
     // infer the address space and would emit flat_* operations
     using GT = PRODM_GLOBAL T;
     constexpr bool kDec = MODE == kDecompose;
-    constexpr int kPE = kDec ? kPL : 1;  // input planes staged only by decompose
-    __shared__ T E[kPE], O[kPE], Wb[kPL], Wb2[kPE];
-    __shared__ double G2[kG2], G2b[kDec ? kG2 : 1];
+    T *const E = lds.E, *const O = lds.O, *const Wb = lds.Wb, *const Wb2 = lds.Wb2;
+    double *const G2 = lds.G2, *const G2b = lds.G2b;
     const int tid = threadIdx.x;
     const uint32_t n0 = P.n[0], n1 = P.n[1], n2 = P.n[2], c0 = P.c[0], c1 = P.c[1], c2 = P.c[2];
     const MarchTile M = march_tile(bt, n0, n1, n2, c0, c1, c2, P.march);
-    const bool act0 = n0 > 1, act1 = n1 > 1, act2 = n2 > 1;
+    const bool act0 = A3 || n0 > 1, act1 = A3 || n1 > 1, act2 = A3 || n2 > 1;
 
     // ---- per-slot constants: slot m of this thread is plane element
     // idx = tid + 256 m at (r1, r2) of the (kN1 x kN2) region; everything
@@ -293,10 +302,9 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt) {
     auto own_plane = [&](int i0) __attribute__((always_inline)) -> bool {
         return !act0 || (i0 >= 2 * M.js0 - 1 && i0 < 2 * (M.js0 + M.tj0) - 1);
     };
-    auto keep_max = [&](bool st, T w) __attribute__((always_inline)) {
-        const T aw = (st && w == w) ? fabs(w) : (T)0;
-        lmax = aw > lmax ? aw : lmax;
-    };
+    // fmax returns the other operand when one is NaN (IEEE maxNum), so NaN
+    // coefficients never enter the level max
+    auto keep_max = [&](bool st, T w) __attribute__((always_inline)) { lmax = fmax(lmax, st ? fabs(w) : (T)0); };
     // even plane i0 staged in Ev: S per slot into s, W into Wout, coarse
     // values into u_c and coefficients into the level buffer
     auto even_plane = [&](int i0, const T *Ev, T *Wout, T (&s)[kPER]) __attribute__((always_inline)) {
@@ -337,7 +345,8 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt) {
             if (kLoadThreads * (m + 1) > kPL && idx >= kPL) continue;
             const uint32_t b = bits >> (4 * m);
             const T I = (sa[m] + sb[m]) * (T)0.125;
-            const T w = (b & 1) ? Ov[idx] - I : (T)0;
+            const T ov = Ov[idx];  // every slot below kPL holds a staged value
+            const T w = (b & 1) ? ov - I : (T)0;
             Wout[idx] = w;
             const bool st = (b & 1) && (b & 8);
             buf_store<T>(rc, st ? offB[m] : kOffNone, w);
@@ -479,9 +488,10 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt) {
     }
 }
 
-template <class T, int MODE, bool ILV>
+template <class T, int MODE, bool ILV, bool A3>
 __global__ __launch_bounds__(kLoadThreads) void k_load(LoadParams<T> P) {
-    load_tile<T, MODE, ILV>(P, xcd_remap(blockIdx.x, load_tiles(P)));
+    __shared__ SweepLds<T, MODE> lds;
+    load_tile<T, MODE, ILV, A3>(P, xcd_remap(blockIdx.x, load_tiles(P)), lds);
 }
 
 // ---- line solves -----------------------------------------------------------------
@@ -1008,11 +1018,14 @@ template <class T>
 __global__ __launch_bounds__(kLoadThreads) void k_load_grouped(const LoadParams<T> *__restrict__ ps,
                                                                const uint32_t *__restrict__ first, uint32_t nent,
                                                                uint32_t total) {
+    __shared__ SweepLds<T, kDecompose> lds;
     for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
         const uint32_t e = group_entry(first, nent, vb);
         const LoadParams<T> &P = ps[e];
         const uint32_t f0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)first[e]);
-        load_tile<T, kDecompose, true>(P, xcd_remap(vb - f0, load_tiles(P)));
+        const uint32_t t = xcd_remap(vb - f0, load_tiles(P));
+        if (P.n[0] > 1 && P.n[1] > 1 && P.n[2] > 1) load_tile<T, kDecompose, true, true>(P, t, lds);
+        else load_tile<T, kDecompose, true, false>(P, t, lds);
         __syncthreads();  // LDS planes are reused by the next tile
     }
 }
@@ -1259,8 +1272,11 @@ int launch_load(LoadParams<T> P, bool ilv, hipStream_t s) {
     while (march > 2 && (uint64_t)t2n * t1n * march_tiles0(P.n[0], P.c[0], march) < want) march /= 2;
     P.march = march;
     const uint32_t grid = t2n * t1n * march_tiles0(P.n[0], P.c[0], march);
-    if (ilv) hipLaunchKernelGGL((k_load<T, MODE, true>), dim3(grid), dim3(kLoadThreads), 0, s, P);
-    else hipLaunchKernelGGL((k_load<T, MODE, false>), dim3(grid), dim3(kLoadThreads), 0, s, P);
+    const bool a3 = P.n[0] > 1 && P.n[1] > 1 && P.n[2] > 1;
+    if (ilv && a3) hipLaunchKernelGGL((k_load<T, MODE, true, true>), dim3(grid), dim3(kLoadThreads), 0, s, P);
+    else if (ilv) hipLaunchKernelGGL((k_load<T, MODE, true, false>), dim3(grid), dim3(kLoadThreads), 0, s, P);
+    else if (a3) hipLaunchKernelGGL((k_load<T, MODE, false, true>), dim3(grid), dim3(kLoadThreads), 0, s, P);
+    else hipLaunchKernelGGL((k_load<T, MODE, false, false>), dim3(grid), dim3(kLoadThreads), 0, s, P);
     PRODM_CHECK_LAUNCH();
     return kOk;
 }
