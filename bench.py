@@ -392,14 +392,15 @@ def cpu_baseline(u_host, dims, L, nb, planes, all_cores=False):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=513)
     ap.add_argument("--levels", type=int, default=6)
     ap.add_argument("--bitplanes", type=int, default=32)
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--graph", action="store_true", help="time HIP graph replays instead of eager launches")
+    ap.add_argument("--graph", action="store_true", help="time HIP graph replays (the default)")
+    ap.add_argument("--eager", action="store_true", help="time eager launches instead of HIP graph replays")
     ap.add_argument("--no-batched", action="store_true", help="skip the 96-field batched refactor report")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched report (one JSON line)")
     ap.add_argument("--no-perbit", action="store_true", help="skip the PerBit encode report")
@@ -411,6 +412,7 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="run only a plain copy kernel of known traffic (for counter scaling) and exit")
     args = ap.parse_args()
+    args.graph = not args.eager  # graph replays unless --eager
     if args.calibrate:
         calibrate_copy()
         return
@@ -505,7 +507,16 @@ def main():
         log("[bench] FAIL: timed retrieval misses the tolerance")
         sys.exit(1)
     s = torch.cuda.current_stream()
-    # the other mode, for the record
+    # the other mode, for the record (warmed first: the first graph call
+    # captures and instantiates the graphs)
+    with torch.cuda.stream(gs):
+        for _ in range(max(1, args.warmup)):
+            if (not args.graph):
+                plan.refactor_graph(u)
+                plan.retrieve_graph(planes, out)
+            else:
+                plan.refactor(u)
+                plan.retrieve(planes, out)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     with torch.cuda.stream(gs):

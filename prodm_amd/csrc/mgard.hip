@@ -201,7 +201,6 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt, S
     uint32_t in_off[kPER];  // decompose / adjoint: byte offset in an input plane
     uint32_t offA[kPER];    // coefficient byte offset in an even plane (coarse slots: u_c offset)
     uint32_t offB[kPER];    // coefficient byte offset in an odd plane
-    int qb[kPER];           // LDS offset of the low interpolation corner
     uint32_t bits = 0;      // 4 bits per slot: valid, odd i1, odd i2, owned (i1, i2)
     const uint32_t n12 = n1 * n2, plane_in = n12 - c1 * c2;
 #pragma unroll
@@ -214,7 +213,6 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt, S
         const bool own1 = !act1 || (i1 >= 2 * M.js1 - 1 && i1 < 2 * (M.js1 + M.tj1) - 1);
         const bool own2 = !act2 || (i2 >= 2 * M.js2 - 1 && i2 < 2 * (M.js2 + M.tj2) - 1);
         bits |= (uint32_t)(valid | (o1 << 1) | (o2 << 2) | ((own1 && own2) << 3)) << (4 * m);
-        qb[m] = valid ? (o1 ? r1 - 1 : r1) * kN2 + (o2 ? r2 - 1 : r2) : 0;
         const uint32_t u1 = valid ? (uint32_t)i1 : 0, u2 = valid ? (uint32_t)i2 : 0;
         in_off[m] = valid ? (uint32_t)(u1 * (uint32_t)P.is[1] + u2) * (uint32_t)sizeof(T) : kOffNone;
         const uint32_t p1 = rpos(u1, c1), p2 = rpos(u2, c2);
@@ -318,7 +316,7 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt, S
             if (kLoadThreads * (m + 1) > kPL && idx >= kPL) continue;
             const uint32_t b = bits >> (4 * m);
             const int d1 = (b & 2) ? 2 * kN2 : 0, d2 = (b & 4) ? 2 : 0;
-            const int q = qb[m];
+            const int q = idx - (d1 >> 1) - (d2 >> 1);  // LDS offset of the low interpolation corner
             const T S = (Ev[q] + Ev[q + d2]) + (Ev[q + d1] + Ev[q + d1 + d2]);
             s[m] = S;
             const T cv = Ev[idx];
