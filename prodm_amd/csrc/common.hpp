@@ -123,6 +123,9 @@ __device__ inline uint64_t readlane_u64(uint64_t v, uint32_t l) {
     uint32_t lo = readlane_u32((uint32_t)v, l), hi = readlane_u32((uint32_t)(v >> 32), l);
     return ((uint64_t)hi << 32) | lo;
 }
+__device__ inline double readlane_f64(double v, uint32_t l) {
+    return __builtin_bit_cast(double, readlane_u64(__builtin_bit_cast(uint64_t, v), l));
+}
 
 __device__ inline int dev_frexp_exp(float x) { int e = 0; frexpf(x, &e); return e; }
 __device__ inline int dev_frexp_exp(double x) { int e = 0; frexp(x, &e); return e; }

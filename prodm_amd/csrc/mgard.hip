@@ -746,11 +746,25 @@ __device__ __forceinline__ void solve_strided_lines(const SolveParams<T> &S, uin
     double v[SEG];
 #pragma unroll
     for (int i = 0; i < SEG; i++) v[i] = (act && b + i < e) ? fg[base + (uint64_t)(b + i) * stride] : 0.0;
+    // The segment's factors, lane i holding node b + i's (and, for the
+    // backward pass, 0 at node m - 1), read back per node with readlane: no
+    // scalar table loads or index tests inside the recurrences.
+    constexpr int NF = (SEG + 63) / 64;  // factor registers per lane
+    double fl[NF], cl[NF];
+#pragma unroll
+    for (int r = 0; r < NF; r++) {
+        const uint32_t nl = b + 64u * r + lane;
+        const bool has = 64u * r + lane < (uint32_t)SEG && nl < e;
+        fl[r] = has ? thomas_factor(S, (const PRODM_GLOBAL double *)S.inv, m, nl) : 0.0;
+        cl[r] = nl == m - 1 ? 0.0 : fl[r];
+    }
+    auto fac = [&](int i) __attribute__((always_inline)) -> double { return readlane_f64(fl[i / 64], i % 64); };
+    auto cfac = [&](int i) __attribute__((always_inline)) -> double { return readlane_f64(cl[i / 64], i % 64); };
     double A = 0.0, Bm = 1.0;
 #pragma unroll
     for (int i = 0; i < SEG; i++) {
         if (b + i < e) {
-            const double iv = thomas_factor(S, inv, m, b + i);
+            const double iv = fac(i);
             A = (v[i] - A) * iv;
             Bm = -Bm * iv;
         }
@@ -764,7 +778,7 @@ __device__ __forceinline__ void solve_strided_lines(const SolveParams<T> &S, uin
 #pragma unroll
     for (int i = 0; i < SEG; i++) {
         if (b + i < e) {
-            y = (v[i] - y) * thomas_factor(S, inv, m, b + i);
+            y = (v[i] - y) * fac(i);
             v[i] = y;
         }
     }
@@ -772,7 +786,7 @@ __device__ __forceinline__ void solve_strided_lines(const SolveParams<T> &S, uin
 #pragma unroll
     for (int i = SEG - 1; i >= 0; i--) {
         if (b + i < e) {
-            const double c = (b + i == m - 1) ? 0.0 : thomas_factor(S, inv, m, b + i);
+            const double c = cfac(i);
             C = v[i] - c * C;
             Dm = -c * Dm;
         }
@@ -788,8 +802,7 @@ __device__ __forceinline__ void solve_strided_lines(const SolveParams<T> &S, uin
 #pragma unroll
     for (int i = SEG - 1; i >= 0; i--) {
         if (b + i < e) {
-            const double c = (b + i == m - 1) ? 0.0 : thomas_factor(S, inv, m, b + i);
-            z = v[i] - c * z;
+            z = v[i] - cfac(i) * z;
             if (act) epilogue(S, base + (uint64_t)(b + i) * stride, ubase + (uint64_t)(b + i) * ustride, z);
         }
     }

@@ -11,7 +11,9 @@ import oracle_bind as ob
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("dims,L", [([33, 17, 65], 3), ([65, 33], 4), ([129], 6), ([1, 17, 33], 3)])
+@pytest.mark.parametrize("dims,L", [([33, 17, 65], 3), ([65, 33], 4), ([129], 6), ([1, 17, 33], 3),
+                                    # lines past the register-resident solves (the sequential solve)
+                                    ([3, 2049], 1), ([2049, 5], 2), ([1153, 3, 5], 1)])
 def test_adjoint_matches_oracle_f64(gpu, dims, L):
     from prodm_amd import mdr
     rng = np.random.default_rng(1)
@@ -31,10 +33,10 @@ def test_adjoint_f32_matches_oracle(gpu):
     assert np.max(np.abs(got - want)) <= 1e-6 * np.max(np.abs(want))
 
 
-def test_dot_product_identity(gpu):
+@pytest.mark.parametrize("dims,L", [([65, 65, 65], 4), ([5, 2049], 1)])
+def test_dot_product_identity(gpu, dims, L):
     """<R x, y> = <x, R^T y> for the GPU recompose R."""
     from prodm_amd import mdr
-    dims, L = [65, 65, 65], 4
     rng = np.random.default_rng(3)
     x = torch.from_numpy(rng.standard_normal(int(np.prod(dims)))).to(gpu)
     y = torch.from_numpy(rng.standard_normal(int(np.prod(dims)))).to(gpu)
