@@ -546,9 +546,28 @@ template <class T> __device__ inline void epilogue(const SolveParams<T> &S, uint
 // only with identity elements of idle lanes, so the result is the same.
 constexpr int kContigLPW = 2;
 template <int W> constexpr uint32_t contig_lines_per_wave() { return kContigLPW * (64 / W); }
-template <class T, int SEG, int W>
-__device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint64_t nlines, uint64_t line0) {
+// Lines of 64-lane groups are staged through wave-private LDS rows: the
+// wave reads and writes each line in 512-byte runs (a lane's segment of
+// consecutive nodes would otherwise spread every load instruction over the
+// whole line), and each lane moves its segment between LDS and registers.
+// LDS operations of one wave complete in order, so the rows need no barrier.
+// The kernel owns the rows (contig_rows_doubles each). The grouped kernels
+// load directly (WPB 0): rows for their 8-wave workgroups cost more
+// occupancy than the staging saves (batched refactor 1.819 -> 1.869 ms).
+#ifndef PRODM_CONTIG_STAGE
+#define PRODM_CONTIG_STAGE 1
+#endif
+template <int SEG, int W, int WPB> constexpr bool contig_staged() {
+    return PRODM_CONTIG_STAGE && W == 64 && WPB > 0 && WPB * SEG <= 48;  // rows of at most 48 KB per workgroup
+}
+template <int SEG, int W, int WPB> constexpr int contig_rows_doubles() {
+    return contig_staged<SEG, W, WPB>() ? WPB * kContigLPW * 64 * SEG : 0;
+}
+template <class T, int SEG, int W, int WPB>
+__device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint64_t nlines, uint64_t line0,
+                                                   double *lds_rows) {
     constexpr int NL = kContigLPW;
+    constexpr bool kStage = contig_staged<SEG, W, WPB>();
     static_assert(W == 16 || W == 32 || W == 64, "lane group width");
     const uint32_t m = S.c[2];
     const uint32_t lane = lane_id(), wv = threadIdx.x / 64;
@@ -592,10 +611,31 @@ __device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint
         return (in[i] && lo[p] != kOffNone) ? lo[p] + (b + i) * 8u : kOffNone;
     };
     double v[NL][SEG];
+    if constexpr (kStage) {
+        // whole line into this wave's LDS row (m <= 64 SEG), then the segments
 #pragma unroll
-    for (int p = 0; p < NL; p++) {
+        for (int p = 0; p < NL; p++) {
+            double t[SEG];
 #pragma unroll
-        for (int i = 0; i < SEG; i++) v[p][i] = buf_load<double>(rl[p], off(p, i));
+            for (int i = 0; i < SEG; i++) {
+                const uint32_t k = 64u * i + lane;
+                t[i] = buf_load<double>(rl[p], k < m ? k * 8u : kOffNone);
+            }
+#pragma unroll
+            for (int i = 0; i < SEG; i++) lds_rows[(wv * NL + p) * 64 * SEG + 64 * i + lane] = t[i];
+        }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int p = 0; p < NL; p++) {
+#pragma unroll
+            for (int i = 0; i < SEG; i++) v[p][i] = in[i] ? lds_rows[(wv * NL + p) * 64 * SEG + b + i] : 0.0;
+        }
+    } else {
+#pragma unroll
+        for (int p = 0; p < NL; p++) {
+#pragma unroll
+            for (int i = 0; i < SEG; i++) v[p][i] = buf_load<double>(rl[p], off(p, i));
+        }
     }
     // forward: y_i = (f_i - y_{i-1}) * inv_i  == affine map y_{i-1} -> y_i
     double A[NL], Bm[NL];
@@ -672,10 +712,30 @@ __device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint
         }
     }
     if (S.epi == kEpiNone) {
+        if constexpr (kStage) {
+            __builtin_amdgcn_wave_barrier();  // every lane has read its segment
 #pragma unroll
-        for (int p = 0; p < NL; p++)
+            for (int p = 0; p < NL; p++) {
 #pragma unroll
-            for (int i = 0; i < SEG; i++) buf_store<double>(rl[p], off(p, i), v[p][i]);
+                for (int i = 0; i < SEG; i++)
+                    if (in[i]) lds_rows[(wv * NL + p) * 64 * SEG + b + i] = v[p][i];
+            }
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int p = 0; p < NL; p++) {
+#pragma unroll
+                for (int i = 0; i < SEG; i++) {
+                    const uint32_t k = 64u * i + lane;
+                    const double t = lds_rows[(wv * NL + p) * 64 * SEG + 64 * i + lane];
+                    buf_store<double>(rl[p], k < m ? k * 8u : kOffNone, t);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int p = 0; p < NL; p++)
+#pragma unroll
+                for (int i = 0; i < SEG; i++) buf_store<double>(rl[p], off(p, i), v[p][i]);
+        }
         return;
     }
 #pragma unroll
@@ -689,7 +749,8 @@ __device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint
 }
 template <class T, int SEG, int W>
 __global__ __launch_bounds__(256) void k_solve_contig(SolveParams<T> S, uint64_t nlines) {
-    solve_contig_lines<T, SEG, W>(S, nlines, (uint64_t)blockIdx.x * 4 * contig_lines_per_wave<W>());
+    __shared__ double rows[contig_rows_doubles<SEG, W, 4>() > 0 ? contig_rows_doubles<SEG, W, 4>() : 1];
+    solve_contig_lines<T, SEG, W, 4>(S, nlines, (uint64_t)blockIdx.x * 4 * contig_lines_per_wave<W>(), rows);
 }
 
 // contiguous-solve classes by line length: class c holds lines of up to
@@ -1044,29 +1105,31 @@ __global__ __launch_bounds__(kLoadThreads) void k_load_grouped(const LoadParams<
 // each entry runs its own contiguous-solve class (lane group width and
 // slots per lane), up to the launch's largest class MAXC
 template <class T, int C>
-__device__ __forceinline__ void contig_class_lines(const SolveEntry<T> &E, uint32_t wpb, uint32_t vbl) {
-    solve_contig_lines<T, kContigSeg[C], kContigW[C]>(E.S, E.nlines,
-                                                        (uint64_t)vbl * wpb * contig_lines_per_wave<kContigW[C]>());
+__device__ __forceinline__ void contig_class_lines(const SolveEntry<T> &E, uint32_t wpb, uint32_t vbl, double *rows) {
+    solve_contig_lines<T, kContigSeg[C], kContigW[C], 0>(
+        E.S, E.nlines, (uint64_t)vbl * wpb * contig_lines_per_wave<kContigW[C]>(), rows);
 }
 template <class T, int C, int MAXC>
-__device__ __forceinline__ void contig_dispatch(int c, const SolveEntry<T> &E, uint32_t wpb, uint32_t vbl) {
+__device__ __forceinline__ void contig_dispatch(int c, const SolveEntry<T> &E, uint32_t wpb, uint32_t vbl,
+                                                double *rows) {
     if constexpr (C < MAXC) {
-        if (c == C) contig_class_lines<T, C>(E, wpb, vbl);
-        else contig_dispatch<T, C + 1, MAXC>(c, E, wpb, vbl);
+        if (c == C) contig_class_lines<T, C>(E, wpb, vbl, rows);
+        else contig_dispatch<T, C + 1, MAXC>(c, E, wpb, vbl, rows);
     } else {
-        contig_class_lines<T, MAXC>(E, wpb, vbl);  // c == MAXC: no entry exceeds the launch's class
+        contig_class_lines<T, MAXC>(E, wpb, vbl, rows);  // c == MAXC: no entry exceeds the launch's class
     }
 }
 template <class T, int MAXC>
 __global__ __launch_bounds__(512) void k_contig_grouped(const SolveEntry<T> *__restrict__ ps,
                                                         const uint32_t *__restrict__ first, uint32_t nent,
                                                         uint32_t total) {
+    double *const rows = nullptr;  // unstaged
     const uint32_t wpb = blockDim.x / 64;
     for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
         const uint32_t e = group_entry(first, nent, vb);
         const SolveEntry<T> E = uniform_load(ps + e);
         const uint32_t vbl = vb - (uint32_t)__builtin_amdgcn_readfirstlane((int)first[e]);
-        contig_dispatch<T, 1, MAXC>(__builtin_amdgcn_readfirstlane(E.cls), E, wpb, vbl);
+        contig_dispatch<T, 1, MAXC>(__builtin_amdgcn_readfirstlane(E.cls), E, wpb, vbl, rows);
     }
 }
 
