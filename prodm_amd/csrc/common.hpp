@@ -123,6 +123,14 @@ __device__ inline uint64_t readlane_u64(uint64_t v, uint32_t l) {
     uint32_t lo = readlane_u32((uint32_t)v, l), hi = readlane_u32((uint32_t)(v >> 32), l);
     return ((uint64_t)hi << 32) | lo;
 }
+// Orders one wave's LDS stores before its later LDS loads for the compiler;
+// the hardware completes a wave's LDS operations in order, so lanes then read
+// what other lanes of the wave wrote (no workgroup barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 __device__ inline double readlane_f64(double v, uint32_t l) {
     return __builtin_bit_cast(double, readlane_u64(__builtin_bit_cast(uint64_t, v), l));
 }

@@ -624,7 +624,7 @@ __device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint
 #pragma unroll
             for (int i = 0; i < SEG; i++) lds_rows[(wv * NL + p) * 64 * SEG + 64 * i + lane] = t[i];
         }
-        __builtin_amdgcn_wave_barrier();
+        wave_lds_sync();
 #pragma unroll
         for (int p = 0; p < NL; p++) {
 #pragma unroll
@@ -713,14 +713,14 @@ __device__ __forceinline__ void solve_contig_lines(const SolveParams<T> &S, uint
     }
     if (S.epi == kEpiNone) {
         if constexpr (kStage) {
-            __builtin_amdgcn_wave_barrier();  // every lane has read its segment
+            wave_lds_sync();  // every lane has read its segment
 #pragma unroll
             for (int p = 0; p < NL; p++) {
 #pragma unroll
                 for (int i = 0; i < SEG; i++)
                     if (in[i]) lds_rows[(wv * NL + p) * 64 * SEG + b + i] = v[p][i];
             }
-            __builtin_amdgcn_wave_barrier();
+            wave_lds_sync();
 #pragma unroll
             for (int p = 0; p < NL; p++) {
 #pragma unroll
