@@ -472,6 +472,10 @@ template <class T> static int plan_retrieve(prodm_mdr_plan *p, const uint8_t *k,
         hipStream_t ss = (split && l == g.L) ? p->side2 : p->side;
         rc = decode_level(l, ss);
         if (!rc && ahead && l >= 1) rc = mgard_recompose_load<T>(src, g, l, fl[l], ss);
+        // the finest level's solves need only its load vector: they run here,
+        // beside the recomposition of the coarser levels, and its
+        // interpolation forms u_c - z
+        if (!rc && ahead && split && l == g.L) rc = mgard_solve_ahead<T>(g, l, fl[l], p->ws, p->ws_bytes, ss);
         if (!rc && hipEventRecord(p->lev_ev[l], ss) != hipSuccess) { set_error("retrieve: event record"); rc = kErrHip; }
     }
     if (rc) {
@@ -486,7 +490,7 @@ template <class T> static int plan_retrieve(prodm_mdr_plan *p, const uint8_t *k,
     };
     const LevelHook hook{&hook_call<decltype(need)>, &need};
     rc = mgard_recompose<T>(src, g, 1, out, st, p->ws, p->ws_bytes, s, nullptr, p->hier != 0, &hook,
-                            ahead ? fl : nullptr);
+                            ahead ? fl : nullptr, ahead && split);
     // every decode is joined (the last level is waited for before its recomposition)
     if (rc) (void)hipStreamWaitEvent(s, p->lev_ev[g.L], 0);
     return rc;

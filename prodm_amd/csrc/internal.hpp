@@ -131,11 +131,17 @@ int mgard_decompose(const T *in, const uint64_t *in_strides, const LevelGeom &g,
                     bool tables_ready = false, bool hier = false, const LevelHook *hook = nullptr);
 // f_levels (optional): per level l the load vector of that level, already
 // computed by mgard_recompose_load (so those sweeps can run ahead on another
-// stream); the recompose then only solves and interpolates.
+// stream); the recompose then only solves and interpolates. top_solved: the
+// finest level's solves also ran ahead (mgard_solve_ahead), so f_levels[L]
+// holds z and the interpolation forms u_c - z itself.
 template <class T>
 int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int tables_ready, T *out, const uint64_t *out_strides,
                     void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm = nullptr, bool hier = false,
-                    const LevelHook *hook = nullptr, double *const *f_levels = nullptr);
+                    const LevelHook *hook = nullptr, double *const *f_levels = nullptr, bool top_solved = false);
+// the line solves of level l on its load vector f_l (from
+// mgard_recompose_load), z left in f_l; they need no coarse values, so they can
+// run before the coarser levels are recomposed
+template <class T> int mgard_solve_ahead(const LevelGeom &g, int l, double *f_l, void *ws, size_t ws_bytes, hipStream_t s);
 // the recompose load sweep of level l (reads only level l's coefficients)
 // into f_l (rows of pitch round16(c2)); bytes of such a buffer
 template <class T> int mgard_recompose_load(const CoefSink<T> &src, const LevelGeom &g, int l, double *f_l, hipStream_t s);

@@ -946,7 +946,10 @@ __global__ __launch_bounds__(256) void k_solve_seq(SolveParams<T> S, uint32_t m,
 constexpr int kI0 = 8, kI1 = 8, kI2 = 64;
 constexpr int kC0 = kI0 / 2 + 1, kC1 = kI1 / 2 + 1, kC2 = kI2 / 2 + 1;
 
-template <class T, bool ILV>
+// SUBZ: the level's solves ran ahead (the load vector P.f holds z on the
+// coarse grid, rows of pitch P.fp) and the coarse values are u_c - z, formed
+// here as each one is staged, as the last solve's epilogue forms them
+template <class T, bool ILV, bool SUBZ = false>
 __global__ __launch_bounds__(256) void k_interp(LoadParams<T> P, const T *__restrict__ uc, T *__restrict__ out,
                                                 uint64_t os0, uint64_t os1) {
     __shared__ T C[kC0 * kC1 * kC2];
@@ -971,6 +974,9 @@ __global__ __launch_bounds__(256) void k_interp(LoadParams<T> P, const T *__rest
         const uint32_t a2 = idx % e2, a1 = (idx / e2) % e1, a0 = idx / (e2 * e1);
         sl[k] = (a0 * kC1 + a1) * kC2 + a2;
         sv[k] = idx < ne ? uc[((uint64_t)(c0s + a0) * c[1] + (c1s + a1)) * c[2] + (c2s + a2)] : (T)0;
+        if (SUBZ && idx < ne)
+            sv[k] = (T)((double)sv[k] -
+                        ((const PRODM_GLOBAL double *)P.f)[((uint64_t)(c0s + a0) * c[1] + (c1s + a1)) * P.fp + (c2s + a2)]);
     }
 #pragma unroll
     for (int k = 0; k < kStage; k++)
@@ -1447,7 +1453,7 @@ template int mgard_decompose<double>(const double *, const uint64_t *, const Lev
 template <class T>
 int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int tables_ready, T *out, const uint64_t *out_strides,
                     void *ws, size_t ws_bytes, hipStream_t s, StageTimer *tm, bool hier, const LevelHook *hook,
-                    double *const *f_levels) {
+                    double *const *f_levels, bool top_solved) {
     const LevelHook none{nullptr, nullptr};
     const LevelHook &need = hook ? *hook : none;
     const WsLayout w = ws_layout(g, sizeof(T));
@@ -1479,7 +1485,8 @@ int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int tables_ready
         double *fl = f_levels ? f_levels[l] : f;
         P.f = fl;
         P.fp = f_pitch(P.c[2]);
-        if (!hier) {
+        const bool subz = !hier && top_solved && f_levels && ilv && l == g.L;
+        if (!hier && !subz) {
             if (!f_levels) {
                 StageScope sc(tm, l == g.L ? kStRecLoadTop : kStRecLoadRest);
                 rc = launch_load<T, kRecompose>(P, ilv, s);
@@ -1497,7 +1504,8 @@ int mgard_recompose(const CoefSink<T> &src, const LevelGeom &g, int tables_ready
         const uint32_t grid = div_up(n[2], kI2) * div_up(n[1], kI1) * div_up(n[0], kI0);
         {
             StageScope sc(tm, l == g.L ? kStRecInterpTop : kStRecInterpRest);
-            if (ilv) hipLaunchKernelGGL((k_interp<T, true>), dim3(grid), dim3(256), 0, s, P, cur, dst, d0, d1);
+            if (subz) hipLaunchKernelGGL((k_interp<T, true, true>), dim3(grid), dim3(256), 0, s, P, cur, dst, d0, d1);
+            else if (ilv) hipLaunchKernelGGL((k_interp<T, true>), dim3(grid), dim3(256), 0, s, P, cur, dst, d0, d1);
             else hipLaunchKernelGGL((k_interp<T, false>), dim3(grid), dim3(256), 0, s, P, cur, dst, d0, d1);
         }
         PRODM_CHECK_LAUNCH();
@@ -1540,10 +1548,20 @@ const double *mgard_adjoint_solution(const LevelGeom &g, int l, void *ws, uint32
 }
 
 template int mgard_recompose<float>(const CoefSink<float> &, const LevelGeom &, int, float *, const uint64_t *, void *,
-                                    size_t, hipStream_t, StageTimer *, bool, const LevelHook *, double *const *);
+                                    size_t, hipStream_t, StageTimer *, bool, const LevelHook *, double *const *, bool);
 template int mgard_recompose<double>(const CoefSink<double> &, const LevelGeom &, int, double *, const uint64_t *,
                                      void *, size_t, hipStream_t, StageTimer *, bool, const LevelHook *,
-                                     double *const *);
+                                     double *const *, bool);
+
+template <class T> int mgard_solve_ahead(const LevelGeom &g, int l, double *f_l, void *ws, size_t ws_bytes, hipStream_t s) {
+    const WsLayout w = ws_layout(g, sizeof(T));
+    if (l < 1 || l > g.L || !f_l || ws_bytes < w.total) { set_error("solve ahead: bad level or workspace"); return kErrArg; }
+    const double *tab = reinterpret_cast<const double *>(static_cast<const char *>(ws) + w.tab);
+    const Factors F = make_factors(g);
+    return solve_all<T>(g, l, f_l, tab, F, kEpiNone, (const T *)nullptr, (T *)nullptr, s);
+}
+template int mgard_solve_ahead<float>(const LevelGeom &, int, double *, void *, size_t, hipStream_t);
+template int mgard_solve_ahead<double>(const LevelGeom &, int, double *, void *, size_t, hipStream_t);
 
 size_t mgard_level_f_bytes(const LevelGeom &g, int l) {
     if (l < 1 || l > g.L) return 0;
