@@ -452,10 +452,11 @@ template <class T> static int plan_retrieve(prodm_mdr_plan *p, const uint8_t *k,
     // orthogonal decomposer, its recompose load sweep into a per-level load
     // vector (it reads only that level's coefficients); the main stream's
     // chain of solves and interpolations waits for level l just before it.
-    // The finest level's decode and load sweep, the bulk of the bytes, start
-    // at once on a second side stream instead of after the coarser levels'
-    // short, latency-bound launches (negabinary: its decode needs no
-    // workspace, so the two side streams share nothing).
+    // The finest level's decode, load sweep and line solves, the bulk of the
+    // bytes, start at once on a second side stream instead of after the
+    // coarser levels' short, latency-bound launches (negabinary: its decode
+    // needs no workspace, so the two side streams share nothing but the
+    // read-only factor tables).
     if ((rc = plan_side(p))) return rc;
     const bool ahead = p->hier == 0 && g.L >= 1;
     double *fl[33] = {nullptr};
