@@ -21,7 +21,12 @@
  *     with a tensor-product mass matrix, coarse nodes first along each dim.
  *   - adjoint (transpose) of recompose, for the QoI extension.
  *
- * Pinning status of each function is documented in oracle/README.md.
+ * Each function is pinned by tests/test_oracle_golden.py (the reference's
+ * own fixtures), tests/test_oracle_ref.py (the compiled reference, where
+ * it builds) and tests/test_oracle_math.py (closed forms, invariants and
+ * brute force). MGARDx's own operation order and rounding cannot be
+ * compared (the library is absent): that part is parity unpinned; the
+ * mathematics above is pinned.
  */
 #ifndef PRODM_MDR_ORACLE_H
 #define PRODM_MDR_ORACLE_H
