@@ -107,14 +107,27 @@ __device__ inline DimTile dim_tile(uint32_t tile, int TJ, uint32_t c, uint32_t n
 // Per fine plane: W (in LDS), coefficient / coarse-value / max stores, the
 // load stencil along dim 2 (LDS, FP64) and dim 1 (registers, one coarse
 // column per thread), and the dim-0 stencil as a running sum over planes.
-// 7 x 33 coarse nodes: coarse extents are 2^k + 1, so 32-wide tiles would
-// leave a one-column tile at the end of every dim (257 = 8 * 32 + 1); the
-// 17 x 69 region still fits five elements per thread.
-constexpr int kM1 = 7, kM2 = 33;
-constexpr int kN1 = 2 * kM1 + 3, kN2 = 2 * kM2 + 3, kPL = kN1 * kN2;
-constexpr int kPER = (kPL + kLoadThreads - 1) / kLoadThreads;   // plane elements per thread
-constexpr int kG2 = kN1 * kM2;
-constexpr int kG2PER = (kG2 + kLoadThreads - 1) / kLoadThreads;
+// Tile shapes: coarse extents are 2^k + 1, so 32-wide tiles would leave a
+// one-column tile at the end of every dim (257 = 8 * 32 + 1).
+//   SweepStd: 7 x 33 coarse nodes, 256 threads; the 17 x 69 region fits
+//     five elements per thread.
+//   SweepWide: 7 x 65 coarse nodes, 512 threads (the fp32 decompose sweep of
+//     levels with many tiles). The same five elements per thread and waves
+//     per SIMD (two workgroups of 54 KB LDS per CU); each fine row is stored
+//     in runs of 65 coefficients instead of 33, and the tile re-reads less
+//     halo. Finest sweep at 513^3 0.378 -> 0.366 ms. The recompose sweep and
+//     the smaller levels (fewer, larger tiles) measured slower with it.
+template <int M1, int M2, int THREADS> struct SweepShape {
+    static constexpr int kM1 = M1, kM2 = M2, kThreads = THREADS;
+    static constexpr int kN1 = 2 * M1 + 3, kN2 = 2 * M2 + 3, kPL = kN1 * kN2;
+    static constexpr int kPER = (kPL + THREADS - 1) / THREADS;  // plane elements per thread
+    static constexpr int kG2 = kN1 * M2;
+    static constexpr int kG2PER = (kG2 + THREADS - 1) / THREADS;
+    static_assert(THREADS % 64 == 0 && M1 * M2 <= THREADS, "a thread per coarse column");
+    static_assert((THREADS / M2) * kG2PER >= kN1, "the dim-2 stencil rows cover the region");
+};
+using SweepStd = SweepShape<7, 33, 256>;
+using SweepWide = SweepShape<7, 65, 512>;
 
 __device__ inline uint32_t xcd_remap(uint32_t bid, uint32_t nblocks) {
     // workgroups are dispatched round-robin over the 8 XCDs: give XCD x the
@@ -140,8 +153,10 @@ struct MarchTile {
     int js0, tj0, js1, tj1, js2, tj2;  // coarse tile
     int rs1, rs2;                      // fine region start (dims 1, 2)
 };
+template <class SH>
 __device__ inline MarchTile march_tile(uint32_t bt, uint32_t n0, uint32_t n1, uint32_t n2, uint32_t c0, uint32_t c1,
                                        uint32_t c2, int kM0) {
+    constexpr int kM1 = SH::kM1, kM2 = SH::kM2;
     const uint32_t n[3] = {n0, n1, n2}, c[3] = {c0, c1, c2};
     const uint32_t t2n = (n[2] == 1) ? 1 : (c[2] + kM2 - 1) / kM2;
     const uint32_t t1n = (n[1] == 1) ? 1 : (c[1] + kM1 - 1) / kM1;
@@ -159,17 +174,18 @@ __device__ inline MarchTile march_tile(uint32_t bt, uint32_t n0, uint32_t n1, ui
 }
 
 // workgroups (coarse tiles) of one load sweep
-template <class T> __host__ __device__ inline uint32_t load_tiles(const LoadParams<T> &P) {
-    const uint32_t t2n = (P.n[2] == 1) ? 1 : (P.c[2] + kM2 - 1) / kM2;
-    const uint32_t t1n = (P.n[1] == 1) ? 1 : (P.c[1] + kM1 - 1) / kM1;
+template <class T, class SH = SweepStd> __host__ __device__ inline uint32_t load_tiles(const LoadParams<T> &P) {
+    const uint32_t t2n = (P.n[2] == 1) ? 1 : (P.c[2] + SH::kM2 - 1) / SH::kM2;
+    const uint32_t t1n = (P.n[1] == 1) ? 1 : (P.c[1] + SH::kM1 - 1) / SH::kM1;
     const uint32_t t0n = march_tiles0(P.n[0], P.c[0], P.march);
     return t0n * t1n * t2n;
 }
 
 // LDS of a sweep: staged fine planes, W planes and dim-2 stencils. The
 // kernels declare it, so that the sweep's forms in one kernel share it.
-template <class T, int MODE> struct SweepLds {
+template <class T, int MODE, class SH = SweepStd> struct SweepLds {
     static constexpr bool kDec = MODE == kDecompose;
+    static constexpr int kPL = SH::kPL, kG2 = SH::kG2;
     static constexpr int kPE = kDec ? kPL : 1;  // input planes staged only by decompose
     T E[kPE], O[kPE], Wb[kPL], Wb2[kPE];
     double G2[kG2], G2b[kDec ? kG2 : 1];
@@ -177,9 +193,11 @@ template <class T, int MODE> struct SweepLds {
 
 // the sweep of coarse tile bt (one workgroup). A3: every dim of the level is
 // longer than 1 (the tests of inactive dims fold away; same arithmetic)
-template <class T, int MODE, bool ILV, bool A3 = false>
-__device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt, SweepLds<T, MODE> &lds) {
+template <class T, int MODE, bool ILV, bool A3 = false, class SH = SweepStd>
+__device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt, SweepLds<T, MODE, SH> &lds) {
     using K = KeyOf<T>;
+    constexpr int kM1 = SH::kM1, kM2 = SH::kM2, kN1 = SH::kN1, kN2 = SH::kN2, kPL = SH::kPL, kPER = SH::kPER;
+    constexpr int kG2 = SH::kG2, kG2PER = SH::kG2PER, kThreads = SH::kThreads;
     // buffer accesses through global-address-space pointers: the parameters
     // may come from memory (grouped launches), where the compiler cannot
     // infer the address space and would emit flat_* operations
@@ -189,11 +207,11 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt, S
     double *const G2 = lds.G2, *const G2b = lds.G2b;
     const int tid = threadIdx.x;
     const uint32_t n0 = P.n[0], n1 = P.n[1], n2 = P.n[2], c0 = P.c[0], c1 = P.c[1], c2 = P.c[2];
-    const MarchTile M = march_tile(bt, n0, n1, n2, c0, c1, c2, P.march);
+    const MarchTile M = march_tile<SH>(bt, n0, n1, n2, c0, c1, c2, P.march);
     const bool act0 = A3 || n0 > 1, act1 = A3 || n1 > 1, act2 = A3 || n2 > 1;
 
     // ---- per-slot constants: slot m of this thread is plane element
-    // idx = tid + 256 m at (r1, r2) of the (kN1 x kN2) region; everything
+    // idx = tid + kThreads m at (r1, r2) of the (kN1 x kN2) region; everything
     // that depends only on (i1, i2) is computed once here, so that a plane
     // step costs a scalar plane offset plus one add per access.
     // offsets are in bytes and 32-bit, so that every access is a scalar
@@ -205,7 +223,7 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt, S
     const uint32_t n12 = n1 * n2, plane_in = n12 - c1 * c2;
 #pragma unroll
     for (int m = 0; m < kPER; m++) {
-        const int idx = tid + kLoadThreads * m;
+        const int idx = tid + kThreads * m;
         const int r1 = idx / kN2, r2 = idx - r1 * kN2;
         const int i1 = M.rs1 + r1, i2 = M.rs2 + r2;
         const bool valid = idx < kPL && i1 >= 0 && i1 < (int)n1 && i2 >= 0 && i2 < (int)n2;
@@ -266,8 +284,8 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt, S
     auto put = [&](T *dst, const T (&reg)[kPER]) __attribute__((always_inline)) {
 #pragma unroll
         for (int m = 0; m < kPER; m++) {
-            const int idx = tid + kLoadThreads * m;
-            if (kLoadThreads * (m + 1) <= kPL || idx < kPL) dst[idx] = reg[m];
+            const int idx = tid + kThreads * m;
+            if (kThreads * (m + 1) <= kPL || idx < kPL) dst[idx] = reg[m];
         }
     };
 
@@ -312,8 +330,8 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt, S
         const auto ru = buf_rsrc((const void *)((GT *)P.uc + (uint64_t)(i0 >> 1) * c1 * c2), own0);
 #pragma unroll
         for (int m = 0; m < kPER; m++) {
-            const int idx = tid + kLoadThreads * m;
-            if (kLoadThreads * (m + 1) > kPL && idx >= kPL) continue;
+            const int idx = tid + kThreads * m;
+            if (kThreads * (m + 1) > kPL && idx >= kPL) continue;
             const uint32_t b = bits >> (4 * m);
             const int d1 = (b & 2) ? 2 * kN2 : 0, d2 = (b & 4) ? 2 : 0;
             const int q = idx - (d1 >> 1) - (d2 >> 1);  // LDS offset of the low interpolation corner
@@ -339,8 +357,8 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt, S
         const auto rc = buf_rsrc((const void *)((GT *)(ILV ? P.coef_ilv : P.coef_full) + coef_base(i0)), own0);
 #pragma unroll
         for (int m = 0; m < kPER; m++) {
-            const int idx = tid + kLoadThreads * m;
-            if (kLoadThreads * (m + 1) > kPL && idx >= kPL) continue;
+            const int idx = tid + kThreads * m;
+            if (kThreads * (m + 1) > kPL && idx >= kPL) continue;
             const uint32_t b = bits >> (4 * m);
             const T I = (sa[m] + sb[m]) * (T)0.125;
             const T ov = Ov[idx];  // every slot below kPL holds a staged value
@@ -352,14 +370,14 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt, S
         }
     };
     // load stencil along dim 2 from Wb into G2[r1][jj2]; this thread's jj2
-    // is fixed (tid % kM2), its rows step by kLoadThreads / kM2
+    // is fixed (tid % kM2), its rows step by kThreads / kM2
     auto stencil2 = [&](const T *Wsrc, double *G) __attribute__((always_inline)) {
-        constexpr int kRowStep = kLoadThreads / kM2;
+        constexpr int kRowStep = kThreads / kM2;
         if (cj2 >= M.tj2 || cj1 >= kRowStep) return;
 #pragma unroll
         for (int m = 0; m < kG2PER; m++) {
             const int r1 = cj1 + kRowStep * m;
-            if (kLoadThreads * (m + 1) > kG2 && r1 >= kN1) continue;
+            if (kThreads * (m + 1) > kG2 && r1 >= kN1) continue;
             const T *row = Wsrc + r1 * kN2;
             double a;
             if (!act2) {
@@ -486,10 +504,10 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt, S
     }
 }
 
-template <class T, int MODE, bool ILV, bool A3>
-__global__ __launch_bounds__(kLoadThreads) void k_load(LoadParams<T> P) {
-    __shared__ SweepLds<T, MODE> lds;
-    load_tile<T, MODE, ILV, A3>(P, xcd_remap(blockIdx.x, load_tiles(P)), lds);
+template <class T, int MODE, bool ILV, bool A3, class SH>
+__global__ __launch_bounds__(SH::kThreads) void k_load(LoadParams<T> P) {
+    __shared__ SweepLds<T, MODE, SH> lds;
+    load_tile<T, MODE, ILV, A3, SH>(P, xcd_remap(blockIdx.x, load_tiles<T, SH>(P)), lds);
 }
 
 // ---- line solves -----------------------------------------------------------------
@@ -1092,6 +1110,7 @@ template <class T> struct Level0Entry {
     uint32_t c0, c1, c2, pad;
 };
 
+static_assert(SweepStd::kThreads == kLoadThreads, "grouped sweeps run the standard tile shape");
 template <class T>
 __global__ __launch_bounds__(kLoadThreads) void k_load_grouped(const LoadParams<T> *__restrict__ ps,
                                                                const uint32_t *__restrict__ first, uint32_t nent,
@@ -1333,14 +1352,14 @@ template <class T> bool sweep_planes_fit(const LoadParams<T> &P) {
     return in_plane < lim && coef_plane < lim;
 }
 
-template <class T, int MODE>
-int launch_load(LoadParams<T> P, bool ilv, hipStream_t s) {
-    if (!sweep_planes_fit(P)) {
-        set_error("a fine plane of this level spans 2 GiB or more; the sweeps address planes with 31-bit offsets");
-        return kErrUnsupported;
-    }
-    const uint32_t t2n = (P.n[2] == 1) ? 1 : div_up(P.c[2], kM2);
-    const uint32_t t1n = (P.n[1] == 1) ? 1 : div_up(P.c[1], kM1);
+// tiles of one sweep of shape SH at march length `march`
+template <class T, class SH> uint32_t sweep_tiles(const LoadParams<T> &P, int march) {
+    const uint32_t t2n = (P.n[2] == 1) ? 1 : div_up(P.c[2], SH::kM2);
+    const uint32_t t1n = (P.n[1] == 1) ? 1 : div_up(P.c[1], SH::kM1);
+    return t2n * t1n * march_tiles0(P.n[0], P.c[0], march);
+}
+
+template <class T, int MODE, class SH> void launch_sweep(LoadParams<T> P, bool ilv, hipStream_t s) {
     // march length: 16 coarse planes per workgroup, shorter when the level
     // has too few tiles to fill the 256 CUs (short marches = more
     // parallelism). Measured at 513^3, 6 levels: the decompose sweep keeps
@@ -1349,14 +1368,33 @@ int launch_load(LoadParams<T> P, bool ilv, hipStream_t s) {
     // the recompose sweep is fastest shortening below 2048.
     int march = 16;
     const uint64_t want = MODE == kDecompose ? 256 : 2048;
-    while (march > 2 && (uint64_t)t2n * t1n * march_tiles0(P.n[0], P.c[0], march) < want) march /= 2;
+    while (march > 2 && sweep_tiles<T, SH>(P, march) < want) march /= 2;
     P.march = march;
-    const uint32_t grid = t2n * t1n * march_tiles0(P.n[0], P.c[0], march);
+    const uint32_t grid = sweep_tiles<T, SH>(P, march);
     const bool a3 = P.n[0] > 1 && P.n[1] > 1 && P.n[2] > 1;
-    if (ilv && a3) hipLaunchKernelGGL((k_load<T, MODE, true, true>), dim3(grid), dim3(kLoadThreads), 0, s, P);
-    else if (ilv) hipLaunchKernelGGL((k_load<T, MODE, true, false>), dim3(grid), dim3(kLoadThreads), 0, s, P);
-    else if (a3) hipLaunchKernelGGL((k_load<T, MODE, false, true>), dim3(grid), dim3(kLoadThreads), 0, s, P);
-    else hipLaunchKernelGGL((k_load<T, MODE, false, false>), dim3(grid), dim3(kLoadThreads), 0, s, P);
+    const dim3 blk(SH::kThreads);
+    if (ilv && a3) hipLaunchKernelGGL((k_load<T, MODE, true, true, SH>), dim3(grid), blk, 0, s, P);
+    else if (ilv) hipLaunchKernelGGL((k_load<T, MODE, true, false, SH>), dim3(grid), blk, 0, s, P);
+    else if (a3) hipLaunchKernelGGL((k_load<T, MODE, false, true, SH>), dim3(grid), blk, 0, s, P);
+    else hipLaunchKernelGGL((k_load<T, MODE, false, false, SH>), dim3(grid), blk, 0, s, P);
+}
+
+template <class T, int MODE>
+int launch_load(LoadParams<T> P, bool ilv, hipStream_t s) {
+    if (!sweep_planes_fit(P)) {
+        set_error("a fine plane of this level spans 2 GiB or more; the sweeps address planes with 31-bit offsets");
+        return kErrUnsupported;
+    }
+    // the wide tiles for fp32 decompose levels that still give 2048
+    // workgroups at full march (513^3: the finest level only); fp64 planes
+    // would take 90 KB of LDS per workgroup
+    bool wide = false;
+    if constexpr (MODE == kDecompose && sizeof(T) == 4) wide = sweep_tiles<T, SweepWide>(P, 16) >= 2048;
+    if (wide) {
+        if constexpr (MODE == kDecompose && sizeof(T) == 4) launch_sweep<T, MODE, SweepWide>(P, ilv, s);
+    } else {
+        launch_sweep<T, MODE, SweepStd>(P, ilv, s);
+    }
     PRODM_CHECK_LAUNCH();
     return kOk;
 }
@@ -1608,10 +1646,8 @@ template <class T> int default_march(const LoadParams<T> &P) {
     // shorter marches while the field's level has few tiles (in a grouped
     // launch the fields' workgroups fill the GPU together; the autotuner
     // picks the march per launch)
-    const uint32_t t2n = (P.n[2] == 1) ? 1 : div_up(P.c[2], kM2);
-    const uint32_t t1n = (P.n[1] == 1) ? 1 : div_up(P.c[1], kM1);
     int march = 16;
-    while (march > 2 && (uint64_t)t2n * t1n * march_tiles0(P.n[0], P.c[0], march) < 2048) march /= 2;
+    while (march > 2 && sweep_tiles<T, SweepStd>(P, march) < 2048) march /= 2;
     return march;
 }
 

@@ -197,6 +197,25 @@ def test_batched_rejects_bad_fields(gpu):
     b.close()
 
 
+def test_batched_equals_plain_at_513(gpu):
+    """One 513^3 float32 field (6 levels): the plan's finest decompose sweep
+    runs 7 x 65-node tiles of 512 threads, the grouped launches the 7 x 33
+    tiles of 256; the coefficients, maxima and planes are bit-identical."""
+    from prodm_amd import mdr
+    from prodm_amd.batched import Batch
+    d, L = [513] * 3, 6
+    print(f"batched vs plain at {d}, L={L}", flush=True)
+    x = bench.make_field_gpu(513)
+    b = Batch([(x, d, L)], 32)
+    b.refactor()
+    p = mdr.Plan(d, L, 32, torch.float32, 0)
+    p.refactor(x)
+    torch.cuda.synchronize()
+    assert_same_results(b.plan(0), p, L, 32, 4, 0, "513^3")
+    p.close()
+    b.close()
+
+
 def test_batched_full_workload(gpu):
     """96 fields (47 2-D, 49 3-D), 203 M points: bit-identical to plain runs"""
     from prodm_amd import mdr

@@ -53,6 +53,29 @@ def _level(plan, l):
     return np.float32(m), e, n
 
 
+def test_config1_coefficients_match_oracle(refactored):
+    """Every level's coefficients against the oracle's decompose + interleave
+    of the same field (fp32 tolerance 1e-6 of the largest coefficient, as for
+    the smaller plan tests); the finest level is the sweep of 7 x 65-node
+    tiles that only fields of this size take."""
+    u, plan = refactored
+    host = u.cpu().numpy()
+    prev = ob.use_library(ob.ORACLE_MT_SO)
+    try:
+        dec = ob.decompose(host, [N] * 3, L)
+        ld = ob.level_dims([N] * 3, L)
+        ref = [ob.interleave(dec, [N] * 3, ld[l], ld[l - 1] if l else np.zeros(3, np.uint32)) for l in range(L + 1)]
+    finally:
+        ob.restore_library(prev)
+    scale = max(float(np.max(np.abs(r))) for r in ref)
+    for l in range(L + 1):
+        c = plan.coefficients(l).cpu().numpy()
+        assert c.size == ref[l].size, l
+        err = float(np.max(np.abs(c.astype(np.float64) - ref[l])))
+        print(f"level {l}: {c.size} coefficients, max |gpu - oracle| = {err:.3e} (scale {scale:.3e})", flush=True)
+        assert err <= 1e-6 * scale, l
+
+
 def test_config1_squared_collector_tables(refactored):
     from prodm_amd import mdr
     _, plan = refactored
