@@ -116,3 +116,28 @@ def test_plan_streams_round_trip(gpu, dims, L, dt, enc, nb):
         np.testing.assert_array_equal(x.view(np.uint8), y.view(np.uint8))
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("dims,L,dt", [
+    ([65, 65, 65], 4, np.float32),
+    ([129, 33, 65], 3, np.float32),
+    ([257, 129], 5, np.float64),
+])
+def test_retrieve_paths_bit_identical(gpu, dims, L, dt):
+    """The plan's retrieve runs the finest level's decode, load sweep and line
+    solves on a second side stream and forms u_c - z in the interpolation;
+    with stage timing on, every launch runs on one stream and the last solve
+    forms u_c - z. Both give the same field bit for bit."""
+    from prodm_amd import mdr
+    u = bumps_field(dims, seed=21, dtype=dt)
+    nb = 32 if dt == np.float32 else 64
+    plan = mdr.Plan(dims, L, nb, TD[dt], 0)
+    plan.refactor(torch.from_numpy(u).to(gpu))
+    for ks in ([nb] * (L + 1), [nb - 2 * l for l in range(L + 1)][::-1]):
+        a = plan.retrieve(ks).clone()
+        plan.profile(True)
+        b = plan.retrieve(ks).clone()
+        plan.profile(False)
+        torch.cuda.synchronize()
+        assert torch.equal(a.view(torch.uint8), b.view(torch.uint8)), ks
+    plan.close()
