@@ -430,12 +430,21 @@ _hip = None
 
 
 def _d2d(dst_tensor, src_ptr, nbytes):
-    """Copy nbytes from a raw device pointer into a tensor (hipMemcpy D2D)."""
+    """Copy nbytes from a raw device pointer into a tensor; returns once the
+    copy is complete. A device-to-device hipMemcpy goes to the null stream
+    and may return before the copy ends, and torch's side streams do not
+    wait for the null stream: without the device synchronization a kernel
+    on such a stream could read the tensor half-copied."""
     global _hip
     if _hip is None:
         _hip = C.CDLL("libamdhip64.so")
         _hip.hipMemcpy.restype = C.c_int
         _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip.hipDeviceSynchronize.restype = C.c_int
+        _hip.hipDeviceSynchronize.argtypes = []
     rc = _hip.hipMemcpy(C.c_void_p(dst_tensor.data_ptr()), C.c_void_p(src_ptr), nbytes, 3)
     if rc != 0:
         raise capi.ProdmError(f"hipMemcpy failed: {rc}")
+    rc = _hip.hipDeviceSynchronize()
+    if rc != 0:
+        raise capi.ProdmError(f"hipDeviceSynchronize after the copy failed: {rc}")
