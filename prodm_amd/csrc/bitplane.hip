@@ -892,13 +892,13 @@ template <class T> struct PbLane {
 // in plane p. Unrolled, so that the words stay in registers.
 template <class T, class Fn> __device__ __forceinline__ void pb_for_planes(const PbLane<T> &P, int nb, Fn &&f) {
     constexpr int W = PbLane<T>::kWords;
-    uint64_t H = 0;
+    uint32_t H = 0;
     static_for<0, W>([&](auto kc) __attribute__((always_inline)) {
         constexpr int k = W - 1 - decltype(kc)::value;
         const int p = nb - 1 - k;
         if (p >= 0) {
-            const uint64_t D = P.w[k];
-            const uint64_t F = D & ~H;
+            const uint32_t D = P.w[k];
+            const uint32_t F = D & ~H;
             H |= D;
             f(p, D, F);
         }
@@ -929,8 +929,8 @@ __global__ __launch_bounds__(kPbTPB) void k_pb_count(const T *__restrict__ x, ui
     P.load(x, ((uint64_t)blockIdx.x * kPbTPB + threadIdx.x) * kPbB, n, nb - level_exp<T>(exp, key), compat,
            ((uintptr_t)x & 15) == 0);
     uint32_t mine = 0;  // lane p: plane p's first-1 flags of the wave
-    pb_for_planes(P, nb, [&](int p, uint64_t, uint64_t F) {
-        const uint32_t s = __ockl_wfred_add_u32((uint32_t)__popcll(F));
+    pb_for_planes(P, nb, [&](int p, uint32_t, uint32_t F) {
+        const uint32_t s = __ockl_wfred_add_u32((uint32_t)__popc(F));
         if (lane == p) mine = s;
     });
     if (lane < nb) bits[(uint64_t)lane * nwv + gw] = pb_wave_valid<T>(n, gw) + mine;
@@ -1026,22 +1026,22 @@ __global__ __launch_bounds__(kPbTPB) void k_pb_write(const T *__restrict__ x, ui
     PbLane<T> P;
     P.load(x, ((uint64_t)blockIdx.x * kPbTPB + threadIdx.x) * B, n, nb - level_exp<T>(exp, key), compat,
            ((uintptr_t)x & 15) == 0);
-    const uint64_t S = P.sg;  // sign bits of the lane's elements
-    pb_for_planes(P, nb, [&](int p, uint64_t D, uint64_t F) __attribute__((always_inline)) {
+    const uint32_t S = P.sg;  // sign bits of the lane's elements
+    pb_for_planes(P, nb, [&](int p, uint32_t D, uint32_t F) __attribute__((always_inline)) {
         const uint64_t o = ((uint64_t)readlane_u32((uint32_t)(my_off >> 32), p) << 32) |
                            readlane_u32((uint32_t)my_off, p);
-        const uint32_t c = (uint32_t)__popcll(F);
+        const uint32_t c = (uint32_t)__popc(F);
         const uint32_t pre = __ockl_wfscan_add_u32(c, false);
         const uint32_t flags = readlane_u32(pre + c, 63);  // the wave's total
         const uint32_t q = (uint32_t)(o & 63) + below + pre;
         // insert each flagged element's sign after its bit, highest first
         uint64_t r = D;
-        uint64_t ff = F;
+        uint32_t ff = F;
         while (ff) {
-            const int j = 63 - __clzll(ff);
-            ff ^= 1ull << j;
+            const int j = 31 - __clz(ff);
+            ff ^= 1u << j;
             const uint64_t low = (2ull << j) - 1;
-            r = (r & low) | (((S >> j) & 1u) << (j + 1)) | ((r & ~low) << 1);
+            r = (r & low) | ((uint64_t)((S >> j) & 1u) << (j + 1)) | ((r & ~low) << 1);
         }
         const uint32_t wi = q >> 6, sh = q & 63;
         const uint64_t w0 = r << sh, w1 = sh ? r >> (64 - sh) : 0;
