@@ -905,6 +905,25 @@ template <class T, class Fn> __device__ __forceinline__ void pb_for_planes(const
     });
 }
 
+// The first-1 counts of planes 2i and 2i + 1 packed in one word (low and
+// high 16 bits: a lane counts at most 32 per plane, a wave at most 2048), so
+// that one wave-wide scan or sum serves two planes. f(i, packed count) for
+// the pairs i in plane order; planes p < 0 (nb below the word count) count 0.
+template <class T, class Fn> __device__ __forceinline__ void pb_for_plane_pairs(const PbLane<T> &P, int nb, Fn &&f) {
+    constexpr int W = PbLane<T>::kWords;
+    uint32_t H = 0, acc = 0;
+    static_for<0, W>([&](auto kc) __attribute__((always_inline)) {
+        constexpr int i = decltype(kc)::value;
+        constexpr int k = W - 1 - i;
+        const bool on = nb - 1 - k >= 0;
+        const uint32_t D = on ? P.w[k] : 0u;
+        const uint32_t c = (uint32_t)__popc(D & ~H);
+        H |= D;
+        if constexpr (i % 2 == 0) acc = c;
+        else f(i / 2, acc | (c << 16));
+    });
+}
+
 // Runs are cut per wave (64 lanes x 32 consecutive elements).
 template <class T> static uint32_t pb_nwg(uint32_t n) { return div_up(n, (uint64_t)kPbTPB * kPbB); }
 template <class T> static uint32_t pb_nwaves(uint32_t n) { return pb_nwg<T>(n) * (kPbTPB / 64); }
@@ -929,9 +948,13 @@ __global__ __launch_bounds__(kPbTPB) void k_pb_count(const T *__restrict__ x, ui
     P.load(x, ((uint64_t)blockIdx.x * kPbTPB + threadIdx.x) * kPbB, n, nb - level_exp<T>(exp, key), compat,
            ((uintptr_t)x & 15) == 0);
     uint32_t mine = 0;  // lane p: plane p's first-1 flags of the wave
-    pb_for_planes(P, nb, [&](int p, uint32_t, uint32_t F) {
-        const uint32_t s = __ockl_wfred_add_u32((uint32_t)__popc(F));
-        if (lane == p) mine = s;
+    constexpr int W = PbLane<T>::kWords;
+    pb_for_plane_pairs(P, nb, [&](int i, uint32_t pc) {
+        const uint32_t s = __ockl_wfred_add_u32(pc);
+        // pair i holds the planes with word indices W - 1 - 2i, W - 2 - 2i
+        const int pa = nb - W + 2 * i, pb = pa + 1;
+        if (lane == pa) mine = s & 0xFFFFu;
+        if (lane == pb) mine = s >> 16;
     });
     if (lane < nb) bits[(uint64_t)lane * nwv + gw] = pb_wave_valid<T>(n, gw) + mine;
 }
@@ -1027,12 +1050,32 @@ __global__ __launch_bounds__(kPbTPB) void k_pb_write(const T *__restrict__ x, ui
     P.load(x, ((uint64_t)blockIdx.x * kPbTPB + threadIdx.x) * B, n, nb - level_exp<T>(exp, key), compat,
            ((uintptr_t)x & 15) == 0);
     const uint32_t S = P.sg;  // sign bits of the lane's elements
+    // first-1 flags of the lanes below and of the wave: for f32 two planes
+    // per scan, computed before the plane loop (16 words); f64's 64 planes
+    // would hold 32 of them, so it scans per plane
+    constexpr int W = PbLane<T>::kWords;
+    constexpr bool kPairs = sizeof(T) == 4;
+    uint32_t pre2[kPairs ? W / 2 : 1], tot2[kPairs ? W / 2 : 1];
+    if constexpr (kPairs)
+        pb_for_plane_pairs(P, nb, [&](int i, uint32_t pc) __attribute__((always_inline)) {
+            pre2[i] = __ockl_wfscan_add_u32(pc, false);
+            tot2[i] = readlane_u32(pre2[i] + pc, 63);
+        });
     pb_for_planes(P, nb, [&](int p, uint32_t D, uint32_t F) __attribute__((always_inline)) {
         const uint64_t o = ((uint64_t)readlane_u32((uint32_t)(my_off >> 32), p) << 32) |
                            readlane_u32((uint32_t)my_off, p);
-        const uint32_t c = (uint32_t)__popc(F);
-        const uint32_t pre = __ockl_wfscan_add_u32(c, false);
-        const uint32_t flags = readlane_u32(pre + c, 63);  // the wave's total
+        uint32_t pre, flags;
+        if constexpr (kPairs) {
+            // p = nb - W + i for the plane order index i, in pair i / 2
+            const int i = p - nb + W;
+            const uint32_t sh16 = (i & 1) ? 16u : 0u;
+            pre = (pre2[i >> 1] >> sh16) & 0xFFFFu;
+            flags = (tot2[i >> 1] >> sh16) & 0xFFFFu;  // the wave's total
+        } else {
+            const uint32_t c = (uint32_t)__popc(F);
+            pre = __ockl_wfscan_add_u32(c, false);
+            flags = readlane_u32(pre + c, 63);
+        }
         const uint32_t q = (uint32_t)(o & 63) + below + pre;
         // insert each flagged element's sign after its bit, highest first
         uint64_t r = D;
