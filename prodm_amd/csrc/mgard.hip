@@ -876,13 +876,50 @@ __device__ __forceinline__ void solve_strided_lines(const SolveParams<T> &S, uin
     double z = 0.0;
     if (gact)
         for (int w = (int)nw - 1; w > (int)wv; w--) z = sA[wbase + w][lane] + sB[wbase + w][lane] * z;
-    // each node's result is stored as soon as it is known (its epilogue
-    // loads do not wait for the end of the segment)
+    // each node's result is stored as soon as it is known
+    if (S.epi == kEpiNone) {
+#pragma unroll
+        for (int i = SEG - 1; i >= 0; i--) {
+            if (b + i < e) {
+                z = v[i] - cfac(i) * z;
+                if (act) ((PRODM_GLOBAL double *)S.f)[base + (uint64_t)(b + i) * stride] = z;
+            }
+        }
+        return;
+    }
+    // With an epilogue, u_c is updated in place: a node's u_c load issued
+    // after the previous node's store could not be moved above it (the same
+    // buffer), and every node would wait a memory round trip inside the
+    // recurrence. The loads run kAhead nodes ahead in a ring of registers:
+    // node i's slot is refilled with node i - kAhead's original value, which
+    // its own store overwrites only kAhead nodes later.
+    using GT = PRODM_GLOBAL T;
+    constexpr int kAhead = SEG < 8 ? SEG : 8;
+    const GT *uin = (const GT *)S.uc_in;
+    GT *uout = (GT *)S.uc_out;
+    // unconditional loads (a lane or node outside the line reads element 0,
+    // unused): a predicated load would sit in a branch that also takes its
+    // value's wait
+    auto uload = [&](int i) __attribute__((always_inline)) -> T {
+        return uin[(act && b + i < e) ? ubase + (uint64_t)(b + i) * ustride : 0];
+    };
+    T ub[kAhead];
+#pragma unroll
+    for (int i = SEG - 1; i >= SEG - kAhead; i--) ub[i % kAhead] = uload(i);
 #pragma unroll
     for (int i = SEG - 1; i >= 0; i--) {
+        const T u = ub[i % kAhead];
+        if (i - kAhead >= 0) {
+            ub[(i - kAhead) % kAhead] = uload(i - kAhead);
+            // keep the load here: the scheduler would otherwise sink it to
+            // its use, where its wait (vmcnt also counts the stores) would
+            // again serialize the nodes
+            __builtin_amdgcn_sched_barrier(0);
+        }
         if (b + i < e) {
             z = v[i] - cfac(i) * z;
-            if (act) epilogue(S, base + (uint64_t)(b + i) * stride, ubase + (uint64_t)(b + i) * ustride, z);
+            if (act)
+                uout[ubase + (uint64_t)(b + i) * ustride] = (T)(S.epi == kEpiAdd ? (double)u + z : (double)u - z);
         }
     }
 }
