@@ -338,7 +338,10 @@ __global__ __launch_bounds__(kFeThreads) void k_fused_errors(const float *__rest
 #pragma unroll
     for (int i = 0; i < kFeEPT; i++) {
         const uint64_t e = base + (uint64_t)i * kFeThreads + threadIdx.x;
-        const float v = e < n ? x[e] : 0.0f;
+        // unconditional load (past n: element 0, discarded), so that the
+        // loads of all elements are in flight together
+        const float x0 = x[e < n ? e : 0];
+        const float v = e < n ? x0 : 0.0f;
         const float sh = ldexpf(v, shift);
         if (ENC == 0) {
             // (int32) shifted as in the reference (in range: |x| < 2^e)
