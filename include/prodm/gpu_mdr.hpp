@@ -58,6 +58,28 @@ class Plan {
 // streams must be planned on to meet its target).
 enum SquaredTables { kNoSquaredTables = 0, kCollectorTables = 1, kEncoderTables = 2 };
 
+// Grow-only pinned host buffer (hipHostMalloc): copies to and from it run at
+// the link's rate instead of through the driver's bounce buffers.
+struct PinnedHostBuffer {
+    uint8_t *p = nullptr;
+    uint64_t cap = 0;
+    uint8_t *get(uint64_t n) {
+        if (n > cap) {
+            if (p) (void)hipHostFree(p);
+            p = nullptr;
+            hip_check(hipHostMalloc((void **)&p, n ? n : 1, hipHostMallocDefault), "hipHostMalloc");
+            cap = n;
+        }
+        return p;
+    }
+    PinnedHostBuffer() = default;
+    PinnedHostBuffer(const PinnedHostBuffer &) = delete;
+    PinnedHostBuffer &operator=(const PinnedHostBuffer &) = delete;
+    ~PinnedHostBuffer() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
 template <class T, class Compressor, class Writer>
 class GpuRefactor {
   public:
@@ -193,23 +215,8 @@ class GpuRefactor {
     Times times() const { return times_; }
 
   private:
-    // grow-only pinned host buffer and per-level events
-    struct Pinned {
-        uint8_t *p = nullptr;
-        uint64_t cap = 0;
-        uint8_t *get(uint64_t n) {
-            if (n > cap) {
-                if (p) (void)hipHostFree(p);
-                p = nullptr;
-                hip_check(hipHostMalloc((void **)&p, n ? n : 1, hipHostMallocDefault), "hipHostMalloc");
-                cap = n;
-            }
-            return p;
-        }
-        ~Pinned() {
-            if (p) (void)hipHostFree(p);
-        }
-    };
+    using Pinned = PinnedHostBuffer;
+    // per-level events
     struct Events {
         std::vector<hipEvent_t> ev;
         void ensure(size_t n) {
@@ -294,7 +301,8 @@ class GpuReconstructor {
             check(prodm_mdr_set_level_max(plan_->get(), l, b, stream_), "set level max");
         }
         held_.assign(L + 1, 0);
-        host_.assign(product(meta_.dims), 0);
+        host_n_ = product(meta_.dims);
+        host_ = reinterpret_cast<T *>(host_buf_->get(host_n_ * sizeof(T)));
         d_out_ = out_.as<T>(product(meta_.dims));
     }
 
@@ -336,8 +344,9 @@ class GpuReconstructor {
         }
         retriever_.release();
         check(prodm_mdr_retrieve(plan_->get(), held_.data(), d_out_, stream_), "retrieve");
-        to_host(host_.data(), d_out_, host_.size(), stream_);
-        return host_.data();
+        // the result into pinned memory (540 MB at 513^3)
+        to_host(host_, d_out_, host_n_, stream_);
+        return host_;
     }
     T *progressive_reconstruct(double tolerance) { return reconstruct(tolerance); }
     const T *device_data() const { return d_out_; }
@@ -379,7 +388,9 @@ class GpuReconstructor {
     MdrMetadata<T> meta_;
     std::shared_ptr<Plan> plan_;
     std::vector<uint8_t> held_;
-    std::vector<T> host_;
+    std::shared_ptr<PinnedHostBuffer> host_buf_ = std::make_shared<PinnedHostBuffer>();
+    T *host_ = nullptr;
+    uint64_t host_n_ = 0;
     std::map<int, std::vector<uint64_t>> hist_;
     hipStream_t stream_ = nullptr;
     DeviceArena out_;
