@@ -447,9 +447,11 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt, S
     T pa[kPER], pb[kPER];
     if constexpr (kDec) {
         // plane i0lo (even), then per step the odd plane i0 and the even
-        // plane i0 + 1: both W planes are built between two barriers, their
-        // dim-2 stencils into G2 / G2b between the next two, then the
-        // columns take them in plane order
+        // plane i0 + 1, two barriers per step. After the first: the columns
+        // of the previous step (from G2 / G2b, in plane order), then this
+        // step's W planes (E / O into Wb / Wb2). After the second: their
+        // dim-2 stencils into G2 / G2b, and the next step's planes from
+        // registers into E / O (their loads issued one step earlier).
         T sprev[kPER];
         fetch(pa, i0lo);
         put(E, pa);
@@ -457,17 +459,16 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt, S
         __syncthreads();
         even_plane(i0lo, E, Wb, sprev);
         __syncthreads();
-        if (!P.hb) {
-            stencil2(Wb, G2);
-            __syncthreads();
-            column(i0lo, G2);
-        }
-#pragma unroll 1
-        for (int i0 = i0lo + 1; i0 <= i0hi; i0 += 2) {
+        if (!P.hb) stencil2(Wb, G2);
+        if (i0lo + 1 <= i0hi) {
             put(O, pa);
             put(E, pb);
-            if (i0 + 2 <= i0hi) { fetch(pa, i0 + 2); fetch(pb, i0 + 3); }
-            __syncthreads();
+            if (i0lo + 3 <= i0hi) { fetch(pa, i0lo + 3); fetch(pb, i0lo + 4); }
+        }
+        __syncthreads();
+        if (!P.hb) column(i0lo, G2);
+#pragma unroll 1
+        for (int i0 = i0lo + 1; i0 <= i0hi; i0 += 2) {
             T scur[kPER];
             even_plane(i0 + 1, E, Wb2, scur);
             odd_plane(i0, O, Wb, sprev, scur);
@@ -477,7 +478,14 @@ __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt, S
             if (!P.hb) {
                 stencil2(Wb, G2);
                 stencil2(Wb2, G2b);
-                __syncthreads();
+            }
+            if (i0 + 2 <= i0hi) {
+                put(O, pa);
+                put(E, pb);
+                if (i0 + 4 <= i0hi) { fetch(pa, i0 + 4); fetch(pb, i0 + 5); }
+            }
+            __syncthreads();
+            if (!P.hb) {
                 column(i0, G2);
                 column(i0 + 1, G2b);
             }
@@ -1148,10 +1156,13 @@ template <class T> struct Level0Entry {
 };
 
 static_assert(SweepStd::kThreads == kLoadThreads, "grouped sweeps run the standard tile shape");
+// fp64: at least three waves per SIMD, which holds it to 168 VGPRs (the
+// fp32 form fits without a bound and schedules better without one)
+template <class T> constexpr int grouped_load_waves() { return sizeof(T) == 8 ? 3 : 1; }
 template <class T>
-__global__ __launch_bounds__(kLoadThreads) void k_load_grouped(const LoadParams<T> *__restrict__ ps,
-                                                               const uint32_t *__restrict__ first, uint32_t nent,
-                                                               uint32_t total) {
+__global__ __launch_bounds__(kLoadThreads) __attribute__((amdgpu_waves_per_eu(grouped_load_waves<T>()))) void
+k_load_grouped(const LoadParams<T> *__restrict__ ps, const uint32_t *__restrict__ first, uint32_t nent,
+               uint32_t total) {
     __shared__ SweepLds<T, kDecompose> lds;
     for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
         const uint32_t e = group_entry(first, nent, vb);
