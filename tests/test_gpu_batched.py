@@ -163,6 +163,39 @@ def test_batched_graph_autotune_and_cache(gpu, tmp_path):
         assert_same_results(b2.plan(f), b.plan(f), L, 32, 4, 0, f"cached field {f}")
 
 
+@pytest.mark.parametrize("dt,nb", [(np.float32, 32), (np.float64, 64)])
+def test_batched_capped_grids(gpu, tmp_path, dt, nb):
+    """Every grouped launch capped to 40 workgroups through a hand-edited
+    shape cache: each workgroup walks many tiles / lines in its persistent
+    loop (the load sweep's tile body then runs in a loop, with the per-slot
+    set-up redone per tile), and every field stays bit-identical to its
+    plain plan."""
+    from prodm_amd.batched import Batch
+    shapes = MIXED[:7]
+    xs = make_inputs(gpu, shapes, dt, 500)
+    fields = [(x, d, L) for x, (d, L) in zip(xs, shapes)]
+    cache = str(tmp_path / "shapes.txt")
+    Batch(fields, nb, TD[dt]).autotune(cache, reps=1)
+    lines = []
+    for ln in open(cache).read().splitlines():
+        if ln.startswith("launch "):
+            _, i, a, _cap = ln.split()
+            ln = f"launch {i} {a} 40"
+        lines.append(ln)
+    open(cache, "w").write("\n".join(lines) + "\n")
+    b = Batch(fields, nb, TD[dt])
+    b.autotune(cache)
+    desc = b.describe().strip().splitlines()
+    assert desc and all("cap=40" in d for d in desc), desc
+    assert any("load" in d and "grid=40" in d for d in desc), desc
+    b.refactor()
+    torch.cuda.synchronize()
+    plans = plain_plans(xs, shapes, nb, dt, 0)
+    elem = 4 if dt == np.float32 else 8
+    for f, (d, L) in enumerate(shapes):
+        assert_same_results(b.plan(f), plans[f], L, nb, elem, 0, f"capped field {f} {d}")
+
+
 def test_batched_retrieve_through_field_plan(gpu):
     from prodm_amd.batched import Batch
     shapes = MIXED[:4]
