@@ -200,7 +200,7 @@ template <class T, int MODE, class SH = SweepStd> struct SweepLds {
 template <class T, int MODE, bool ILV, bool A3 = false, class SH = SweepStd, bool LOOPED = false>
 __device__ __forceinline__ void load_tile(const LoadParams<T> &P, uint32_t bt, SweepLds<T, MODE, SH> &lds) {
     using K = KeyOf<T>;
-    constexpr int kM1 = SH::kM1, kM2 = SH::kM2, kN1 = SH::kN1, kN2 = SH::kN2, kPL = SH::kPL, kPER = SH::kPER;
+    constexpr int kM2 = SH::kM2, kN1 = SH::kN1, kN2 = SH::kN2, kPL = SH::kPL, kPER = SH::kPER;
     constexpr int kG2 = SH::kG2, kG2PER = SH::kG2PER, kThreads = SH::kThreads;
     // buffer accesses through global-address-space pointers: the parameters
     // may come from memory (grouped launches), where the compiler cannot
@@ -827,10 +827,7 @@ __device__ __forceinline__ void solve_strided_lines(const SolveParams<T> &S, uin
         ustride = (uint64_t)c1 * c2;
         if (act) { base = r * fp + j2; ubase = r * c2 + j2; }
     }
-    // the factors are read-only while a solve runs: a constant-space pointer
-    // lets their wave-uniform loads go through the scalar cache
     const PRODM_GLOBAL double *fg = (const PRODM_GLOBAL double *)S.f;
-    const PRODM_CONST double *inv = (const PRODM_CONST double *)S.inv;
     const uint32_t seg = (m + nw - 1) / nw;
     // wave-uniform segment bounds: the factor loads become scalar loads
     const uint32_t b = (uint32_t)__builtin_amdgcn_readfirstlane((int)(wv * seg));
